@@ -1,0 +1,330 @@
+"""Python mirror of the CUNA interface over the C-ABI (libcunaf / libcunad).
+
+Function names, argument order and meaning follow the reference's inc/cuna.h,
+inc/cuna_filter.h and inc/cuna_typedefs.h; device buffers are torch tensors
+(PyTorch provides device memory and streams), host buffers numpy arrays.
+Errors raise CunaError instead of terminating the process.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import CunaError, check, load
+
+_PREC = {np.dtype(np.float32): "f", np.dtype(np.float64): "d"}
+
+
+def _prec_of(arr) -> str:
+    import torch
+
+    if isinstance(arr, torch.Tensor):
+        dt = {torch.float32: np.dtype(np.float32), torch.float64: np.dtype(np.float64)}.get(arr.dtype)
+    else:
+        dt = np.asarray(arr).dtype
+    if dt not in _PREC:
+        raise TypeError("x must be float32 (libcunaf) or float64 (libcunad)")
+    return _PREC[dt]
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("device buffer expected (a CUDA/HIP torch tensor)")
+    if not t.is_contiguous():
+        raise ValueError("contiguous tensor expected")
+    return t.data_ptr()
+
+
+def _stream(stream) -> int | None:
+    import torch
+
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def complex_dtype(prec: str):
+    import torch
+
+    return torch.complex64 if prec == "f" else torch.complex128
+
+
+def real_dtype(prec: str):
+    import torch
+
+    return torch.float32 if prec == "f" else torch.float64
+
+
+# ------------------------------------------------------ reference API --
+def cunfft_adjoint(x, f, ng: int, flags: int = 0) -> np.ndarray:
+    """inc/cuna.h:33-34: host arrays in, host spectrum (ng complex) out."""
+    x = np.ascontiguousarray(x)
+    prec = _prec_of(x)
+    R = np.float32 if prec == "f" else np.float64
+    x = x.astype(R, copy=False)
+    L = load(prec)
+    RP = ctypes.POINTER(_lib.real_type(prec))
+    fp = None
+    if f is not None:
+        f = np.ascontiguousarray(f, dtype=R)
+        fp = f.ctypes.data_as(RP)
+    out = np.zeros(ng, dtype=np.complex64 if prec == "f" else np.complex128)
+    L.cunfft_adjoint(x.ctypes.data_as(RP), fp, out.ctypes.data, len(x), ng, flags)
+    check(L)
+    return out
+
+
+class FilterProperties:
+    """Host + device filter_properties from generate_filter_properties
+    (inc/cuna_filter.h:47-49).  `device` is the pointer the raw transforms take."""
+
+    def __init__(self, x, n: int, ng: int):
+        self.prec = _prec_of(x)
+        self.L = load(self.prec)
+        R, _, FP, _ = _lib.make_types(self.prec)
+        self._FP = FP
+        h = ctypes.c_void_p()
+        d = ctypes.c_void_p()
+        self.L.generate_filter_properties(_ptr(x), n, ng, ctypes.byref(h), ctypes.byref(d))
+        check(self.L)
+        self.host, self.device = h.value, d.value
+
+    @property
+    def fields(self):
+        return self._FP.from_address(self.host)
+
+    def free(self):
+        if self.device is not None:
+            self.L.free_filter_properties(self.device, self.host)
+            self.host = self.device = None
+            check(self.L)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def generate_filter_properties(x, n: int, ng: int) -> FilterProperties:
+    return FilterProperties(x, n, ng)
+
+
+def generate_pinned_filter_properties(x, n: int, ng: int, h_fprops, d_fprops, stream=None):
+    """inc/cuna_filter.h:37-40; h_fprops is a ctypes filter_properties whose
+    E1/E2/E3 device arrays the caller allocated, d_fprops a device pointer."""
+    prec = _prec_of(x)
+    L = load(prec)
+    L.generate_pinned_filter_properties(_ptr(x), n, ng, ctypes.addressof(h_fprops), d_fprops,
+                                        _stream(stream))
+    check(L)
+
+
+def cunfft_adjoint_raw(x, f_data, f_hat, n: int, ng: int, fprops: FilterProperties):
+    """inc/cuna.h:48-49 (null stream)."""
+    L = load(_prec_of(x))
+    L.cunfft_adjoint_raw(_ptr(x), _ptr(f_data), _ptr(f_hat), n, ng,
+                         fprops.device if isinstance(fprops, FilterProperties) else fprops)
+    check(L)
+
+
+def cunfft_adjoint_raw_async(x, f_data, f_hat, n: int, ng: int, fprops, stream=None):
+    """inc/cuna.h:51-53."""
+    L = load(_prec_of(x))
+    L.cunfft_adjoint_raw_async(_ptr(x), _ptr(f_data), _ptr(f_hat), n, ng,
+                               fprops.device if isinstance(fprops, FilterProperties) else fprops,
+                               _stream(stream))
+    check(L)
+
+
+def cunfft_adjoint_raw_async_bootstrap(x, f_data, f_hat, n: int, ng: int, nbs: int, fprops,
+                                       stream=None, seed: int = 0):
+    """src/cuna_kernel.cu:390-443 (the defined 9-argument form)."""
+    L = load(_prec_of(x))
+    L.cunfft_adjoint_raw_async_bootstrap(
+        _ptr(x), _ptr(f_data), _ptr(f_hat), n, ng, nbs,
+        fprops.device if isinstance(fprops, FilterProperties) else fprops, _stream(stream),
+        seed & 0xFFFFFFFF)
+    check(L)
+
+
+class CunaPlan:
+    """init_plan / cunfft_adjoint_from_plan / free_plan (inc/cuna_typedefs.h:151-155,
+    inc/cuna.h:41) over host arrays."""
+
+    def __init__(self, x, f, ng: int, flags: int = 0):
+        x = np.ascontiguousarray(x)
+        self.prec = _prec_of(x)
+        R = np.float32 if self.prec == "f" else np.float64
+        self.L = load(self.prec)
+        _, _, _, PlanT = _lib.make_types(self.prec)
+        RP = ctypes.POINTER(_lib.real_type(self.prec))
+        self._x = x.astype(R, copy=False)
+        self._f = None if f is None else np.ascontiguousarray(f, dtype=R)
+        self._buf = ctypes.create_string_buffer(ctypes.sizeof(PlanT))
+        self.L.init_plan(ctypes.addressof(self._buf), self._x.ctypes.data_as(RP),
+                         None if self._f is None else self._f.ctypes.data_as(RP), len(self._x), ng,
+                         flags)
+        check(self.L)
+        self.plan = PlanT.from_buffer(self._buf)
+        self.ng, self.flags = ng, flags
+        self._live = True
+
+    def execute(self):
+        self.L.cunfft_adjoint_from_plan(ctypes.addressof(self._buf))
+        check(self.L)
+
+    def f_hat(self) -> np.ndarray:
+        dt = np.complex64 if self.prec == "f" else np.complex128
+        return np.ctypeslib.as_array(
+            ctypes.cast(self.plan.f_hat, ctypes.POINTER(_lib.real_type(self.prec))),
+            shape=(2 * self.ng,)).view(dt).copy()
+
+    def f_hat_win(self) -> np.ndarray:
+        dt = np.complex64 if self.prec == "f" else np.complex128
+        return np.ctypeslib.as_array(
+            ctypes.cast(self.plan.f_hat_win, ctypes.POINTER(_lib.real_type(self.prec))),
+            shape=(2 * self.ng,)).view(dt).copy()
+
+    def free(self):
+        # free_plan also frees the struct itself (src/utils.cu:275), so hand it
+        # a malloc'd copy and keep our buffer
+        if self._live:
+            libc = ctypes.CDLL(None)
+            libc.malloc.restype = ctypes.c_void_p
+            libc.malloc.argtypes = [ctypes.c_size_t]
+            p = libc.malloc(ctypes.sizeof(self._buf))
+            ctypes.memmove(p, ctypes.addressof(self._buf), ctypes.sizeof(self._buf))
+            self.L.free_plan(p)
+            self._live = False
+            check(self.L)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------- extensions --
+class NFFTPlan:
+    """Plan of include/cuna_nfft.h.
+
+    NFFTPlan(x, M)                      accurate adjoint NFFT of points x
+    NFFTPlan(x, M, mode="compat")       the reference formula (M = ng)
+    NFFTPlan(t=t, df=df, M=M)           FP64 times, frequencies k*df
+    NFFTPlan(x, M, offsets=[...])       batch of curves (CSR offsets, host)
+    Precision follows x's dtype (float32 -> libcunaf, float64 -> libcunad);
+    for times plans pass dtype.
+    """
+
+    def __init__(self, x=None, M: int = 0, *, t=None, df: float | None = None, offsets=None,
+                 mode: str = "accurate", sigma: int = 2, m: int | None = None,
+                 tile_log2: int = 0, window2: bool = False, dtype=None, stream=None):
+        import torch
+
+        if x is not None:
+            self.prec = _prec_of(x)
+        else:
+            self.prec = "d" if dtype in (torch.float64, np.float64, "d") else "f"
+        self.L = load(self.prec)
+        p = _lib.NfftParams()
+        self.L.cuna_nfft_default_params(ctypes.addressof(p))
+        p.mode = _lib.CUNA_MODE_COMPAT if mode == "compat" else _lib.CUNA_MODE_ACCURATE
+        p.sigma = sigma
+        if m is not None:
+            p.m = m
+        p.tile_log2 = tile_log2
+        p.flags = _lib.CUNA_PLAN_WINDOW2 if window2 else 0
+        h = ctypes.c_void_p()
+        s = _stream(stream)
+        if t is not None:
+            if t.dtype != torch.float64:
+                raise TypeError("times must be float64")
+            st = self.L.cuna_nfft_plan_create_times(ctypes.byref(h), _ptr(t), t.numel(),
+                                                    float(df), M, ctypes.addressof(p), s)
+            self.n = t.numel()
+        elif offsets is not None:
+            off = np.ascontiguousarray(offsets, dtype=np.int32)
+            st = self.L.cuna_nfft_plan_create_batch(ctypes.byref(h), _ptr(x), off.ctypes.data,
+                                                    len(off) - 1, M, ctypes.addressof(p), s)
+            self.n = int(off[-1])
+        else:
+            st = self.L.cuna_nfft_plan_create(ctypes.byref(h), _ptr(x), x.numel(), M,
+                                              ctypes.addressof(p), s)
+            self.n = x.numel()
+        check(self.L, st)
+        self.h = h.value
+        self.params = p
+        info = self.info()
+        self.M, self.G, self.ncurves = info.M, info.G, info.ncurves
+        self.mode = mode
+
+    def info(self) -> _lib.NfftInfo:
+        inf = _lib.NfftInfo()
+        check(self.L, self.L.cuna_nfft_plan_info(self.h, ctypes.addressof(inf)))
+        return inf
+
+    def _out(self, rows: int):
+        import torch
+
+        return torch.empty((rows, self.M), dtype=complex_dtype(self.prec), device="cuda")
+
+    def adjoint(self, y=None, out=None, stream=None):
+        if out is None:
+            out = self._out(self.ncurves)
+        check(self.L, self.L.cuna_nfft_adjoint(self.h, _ptr(y), _ptr(out), _stream(stream)))
+        return out
+
+    def bootstrap(self, y, nbs: int, seed: int = 0, window: bool = False, stream=None):
+        out_y = self._out(self.ncurves * nbs)
+        out_w = self._out(self.ncurves * nbs) if window else None
+        check(self.L, self.L.cuna_nfft_bootstrap(self.h, _ptr(y), nbs, seed & 0xFFFFFFFF,
+                                                  _ptr(out_y), _ptr(out_w), _stream(stream)))
+        return (out_y, out_w) if window else out_y
+
+    def backward(self, y, grad_out, want_y: bool = True, want_x: bool = True, stream=None):
+        import torch
+
+        rd = real_dtype(self.prec)
+        dy = torch.empty(self.n, dtype=rd, device="cuda") if want_y else None
+        dx = torch.empty(self.n, dtype=rd, device="cuda") if want_x else None
+        g = grad_out.contiguous()
+        check(self.L, self.L.cuna_nfft_backward(self.h, _ptr(y), _ptr(g), _ptr(dy), _ptr(dx),
+                                                 _stream(stream)))
+        return dy, dx
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cuna_nfft_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fft(data, sign: int = +1, stream=None):
+    """In-place unnormalised DFT along the last axis (power-of-two length)."""
+    import torch
+
+    prec = "f" if data.dtype == torch.complex64 else "d"
+    L = load(prec)
+    n = data.shape[-1]
+    check(L, L.cuna_fft(_ptr(data), n, data.numel() // n, sign, _stream(stream)))
+    return data
+
+
+__all__ = [
+    "CunaError", "cunfft_adjoint", "FilterProperties", "generate_filter_properties",
+    "generate_pinned_filter_properties", "cunfft_adjoint_raw", "cunfft_adjoint_raw_async",
+    "cunfft_adjoint_raw_async_bootstrap", "CunaPlan", "NFFTPlan", "fft",
+]

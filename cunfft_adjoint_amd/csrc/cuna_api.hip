@@ -1,0 +1,912 @@
+// cuna_api.hip -- plans and the extern "C" API of libcunaf.so / libcunad.so.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "cuna.h"
+#include "cuna_filter.h"
+#include "cuna_nfft.h"
+#include "cuna_plan.hpp"
+
+#define CUNA_API extern "C" __attribute__((visibility("default")))
+
+namespace cuna {
+
+// ----------------------------------------------------------------- errors --
+namespace {
+thread_local int g_err = CUNA_SUCCESS;
+thread_local std::string g_msg;
+bool g_abort = false;
+}  // namespace
+
+void set_error(int code, const std::string &msg) {
+    g_err = code;
+    g_msg = msg;
+    if (code != CUNA_SUCCESS && g_abort) {
+        fprintf(stderr, "CUNA ERROR: %s\n", msg.c_str());
+        exit(code);
+    }
+}
+void clear_error() {
+    g_err = CUNA_SUCCESS;
+    g_msg.clear();
+}
+bool abort_on_error() { return g_abort; }
+[[noreturn]] void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+template <class F>
+static int guarded(F &&f) {
+    try {
+        clear_error();
+        f();
+        return CUNA_SUCCESS;
+    } catch (const Error &e) {
+        set_error(e.code, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error(CUNA_ERROR_OUT_OF_MEMORY, "host allocation failed");
+        return CUNA_ERROR_OUT_OF_MEMORY;
+    } catch (...) {
+        set_error(CUNA_ERROR_HIP, "unexpected exception");
+        return CUNA_ERROR_HIP;
+    }
+}
+
+using R = dTyp;
+
+// ------------------------------------------------------------- helpers --
+__global__ void k_boot_counts(const int *coff, int ncurves, long n, int nbs, uint32_t seed,
+                              int *counts) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nbs) return;
+    const int s = (int)(t / n);
+    const long i = t % n;
+    int lo = 0, hi = ncurves;  // curve c with coff[c] <= i < coff[c+1]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (coff[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int c0 = coff[lo];
+    const uint32_t nc = (uint32_t)(coff[lo + 1] - c0);
+    const uint32_t p = (uint32_t)(i - c0);
+    const int d = c0 + (int)(xorshift32(wang_hash(seed + (uint32_t)s * nc + p)) % nc);
+    atomicAdd(&counts[(long)s * n + d], 1);
+}
+
+template <typename T>
+__global__ void k_scale2(const T *in, T *out, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = T(2) * in[i];
+}
+
+__global__ void k_fill_int(int *p, long n, int v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------ TiledPoints --
+template <typename T>
+void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int G_, int tlog_,
+                           int ntaps_, bool periodic_, bool keep_eps, hipStream_t s) {
+    ncurves = (int)offsets.size() - 1;
+    coff_h = offsets;
+    n = offsets.back();
+    G = G_;
+    ntaps = ntaps_;
+    olo = -((ntaps - 1) / 2);
+    periodic = periodic_;
+    // tile: as requested, no larger than the grid, and few enough tiles that
+    // the per-curve key counters fit in LDS
+    // 4 private LDS copies of a tile must fit in 128 KB
+    const int tmax = sizeof(T) == 4 ? 13 : 12;
+    tlog = std::min(std::min(tlog_, tmax), ilog2(G));
+    while ((G >> tlog) > 8192) tlog++;
+    require(tlog <= tmax, "grid too large for the spread tiles", CUNA_ERROR_UNSUPPORTED);
+    ntiles = G >> tlog;
+    if (ntiles > 1 && (1 << tlog) < 2 * ntaps) {
+        tlog = ilog2(G);
+        ntiles = 1;
+    }
+    Kc = 3 * ntiles;
+
+    // chunk table: chunks of at most 2048 points, never crossing a curve
+    constexpr int CH = 2048;
+    std::vector<int4> chunks;
+    std::vector<long> mbase(ncurves);
+    std::vector<int> nch(ncurves);
+    long hist_len = 0;
+    for (int c = 0; c < ncurves; ++c) {
+        int nc = offsets[c + 1] - offsets[c];
+        require(nc >= 0, "curve offsets must be non-decreasing");
+        int k = (int)ceil_div(nc, CH);
+        nch[c] = k;
+        mbase[c] = hist_len;
+        for (int q = 0; q < k; ++q) {
+            int st = offsets[c] + q * CH;
+            chunks.push_back(make_int4(c, st, std::min(CH, offsets[c + 1] - st), q));
+        }
+        hist_len += (long)Kc * k;
+    }
+    coff.alloc(sizeof(int) * offsets.size());
+    rec.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1));
+    perm.alloc(sizeof(int) * std::max<long>(n, 1));
+    bucket.alloc(sizeof(int) * (size_t)ncurves * (Kc + 1));
+    if (keep_eps) {
+        eps_orig.alloc(sizeof(T) * std::max<long>(n, 1));
+        A.eps_orig = eps_orig.as<T>();
+    }
+    CUNA_HIP(hipMemcpyAsync(coff.ptr, coff_h.data(), sizeof(int) * coff_h.size(),
+                            hipMemcpyHostToDevice, s));
+    // temporaries, stream-ordered
+    DevBuf d_chunks, d_mbase, d_nch, keys, rec_tmp, hist, bsum;
+    d_chunks.alloc(sizeof(int4) * std::max<size_t>(chunks.size(), 1), s);
+    d_mbase.alloc(sizeof(long) * ncurves, s);
+    d_nch.alloc(sizeof(int) * ncurves, s);
+    keys.alloc(sizeof(int) * std::max<long>(n, 1), s);
+    rec_tmp.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1), s);
+    hist.alloc(sizeof(int) * std::max<long>(hist_len, 1), s);
+    bsum.alloc(sizeof(int) * ceil_div(std::max<long>(hist_len, 1), kScanTile), s);
+    if (!chunks.empty())
+        CUNA_HIP(hipMemcpyAsync(d_chunks.ptr, chunks.data(), sizeof(int4) * chunks.size(),
+                                hipMemcpyHostToDevice, s));
+    CUNA_HIP(hipMemcpyAsync(d_mbase.ptr, mbase.data(), sizeof(long) * ncurves,
+                            hipMemcpyHostToDevice, s));
+    CUNA_HIP(hipMemcpyAsync(d_nch.ptr, nch.data(), sizeof(int) * ncurves, hipMemcpyHostToDevice,
+                            s));
+    A.G = G;
+    A.tlog = tlog;
+    A.ntiles = ntiles;
+    A.Kc = Kc;
+    A.olo = olo;
+    A.ohi = olo + ntaps - 1;
+    A.periodic = periodic;
+    SortWork<T> W;
+    W.chunks = d_chunks.as<int4>();
+    W.mbase = d_mbase.as<long>();
+    W.nch = d_nch.as<int>();
+    W.coff = coff.as<int>();
+    W.nchunks = (int)chunks.size();
+    W.ncurves = ncurves;
+    W.hist_len = hist_len;
+    W.keys = keys.as<int>();
+    W.rec_tmp = rec_tmp.as<PointRec<T>>();
+    W.hist = hist.as<int>();
+    W.bsum = bsum.as<int>();
+    if (n > 0 || A.E3) {
+        if (chunks.empty()) {
+            // no points: buckets all empty
+            long nbk = (long)ncurves * (Kc + 1);
+            hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbk, 256)), dim3(256), 0, s,
+                               bucket.as<int>(), nbk, 0);
+            CUNA_LAUNCH_CHECK();
+        } else {
+            launch_sort<T>(A, W, rec.as<PointRec<T>>(), perm.as<int>(), bucket.as<int>(), s);
+        }
+    }
+    // the host tables stay alive with the plan: the copies above are async
+    chunks_h = std::move(chunks);
+    mbase_h = std::move(mbase);
+    nch_h = std::move(nch);
+}
+
+// --------------------------------------------------------------- Plan --
+template <typename T>
+SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
+    SpreadArgs<T> A{};
+    A.rec = P.rec.template as<PointRec<T>>();
+    A.perm = P.perm.template as<int>();
+    A.bucket = P.bucket.template as<int>();
+    A.coff = P.coff.template as<int>();
+    A.G = P.G;
+    A.tlog = P.tlog;
+    A.ntiles = P.ntiles;
+    A.Kc = P.Kc;
+    A.olo = P.olo;
+    A.ntaps = P.ntaps;
+    A.nrep = 1;
+    A.periodic = P.periodic ? 1 : 0;
+    A.wrap1 = (P.periodic && P.ntiles == 1) ? 1 : 0;
+    A.bi = T(1) / beta;
+    A.wscale = wscale;
+    for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
+    return A;
+}
+
+template <typename T>
+void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s) const {
+    FftShape sh;
+    sh.lgN = lgN;
+    sh.batch = batch;
+    if (spec.mode == CUNA_MODE_ACCURATE) {
+        SinkAccurate<T> sk;
+        sk.out = out;
+        sk.M = M;
+        sk.ostride = M;
+        sk.invG = T(1) / (T)G;
+        const double pi = 3.14159265358979323846;
+        sk.coef = (T)(pi * pi * (double)beta);
+        sk.pref = (T)(1.0 / std::sqrt(pi * (double)beta));
+        fft_real_forward<T>(sh, grid_ptr, sk, s);
+    } else {
+        SinkCompat<T> sk;
+        sk.out = out;
+        sk.G = G;
+        sk.ostride = G;
+        sk.invG = T(1) / (T)G;
+        sk.qb = b_c * T(0.25);
+        sk.normfac = normfac_c;
+        fft_real_forward<T>(sh, grid_ptr, sk, s);
+    }
+}
+
+template <typename T>
+void Plan<T>::adjoint(const T *y, cplx<T> *out, hipStream_t s) {
+    require(out != nullptr, "output pointer is NULL");
+    const int nc = ncurves();
+    grid.ensure(sizeof(T) * (size_t)G * nc);
+    SpreadArgs<T> A = spread_args(pts);
+    A.y = y;
+    A.grid = grid.as<T>();
+    A.grid_stride = G;
+    launch_spread<T>(A, ntaps, VAL_PLAIN, nc, s);
+    run_fft(grid.as<T>(), out, nc, s);
+}
+
+template <typename T>
+void Plan<T>::bootstrap_compat(const T *y, cplx<T> *out, int nbs, uint32_t seed, hipStream_t s) {
+    require(spec.mode == CUNA_MODE_COMPAT && ncurves() == 1, "compat bootstrap needs a compat plan");
+    require(nbs > 0, "nbs must be positive");
+    require(out != nullptr, "output pointer is NULL");
+    grid.ensure(sizeof(T) * (size_t)G * nbs);
+    SpreadArgs<T> A = spread_args(pts);
+    A.y = y;
+    A.grid = grid.as<T>();
+    A.grid_stride = G;
+    A.nrep = nbs;
+    A.seed = seed;
+    A.eps_orig = pts.eps_orig.template as<T>();
+    launch_spread<T>(A, ntaps, VAL_COMPAT_BOOT, nbs, s);
+    run_fft(grid.as<T>(), out, nbs, s);
+}
+
+template <typename T>
+void Plan<T>::bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx<T> *out_w,
+                        hipStream_t s) {
+    require(spec.mode == CUNA_MODE_ACCURATE, "bootstrap needs an accurate-mode plan");
+    require(nbs > 0, "nbs must be positive");
+    require(out_w == nullptr || has_pts2, "window output needs a plan made with CUNA_PLAN_WINDOW2");
+    const int nc = ncurves();
+    const long n_ = n();
+    counts.ensure(sizeof(int) * (size_t)std::max<long>(n_, 1) * nbs);
+    long tot = n_ * nbs;
+    hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(std::max(tot, 1L), 256)), dim3(256), 0, s,
+                       counts.as<int>(), tot, 0);
+    CUNA_LAUNCH_CHECK();
+    if (tot > 0) {
+        hipLaunchKernelGGL(k_boot_counts, dim3(ceil_div(tot, 256)), dim3(256), 0, s,
+                           pts.coff.template as<int>(), nc, n_, nbs, seed, counts.as<int>());
+        CUNA_LAUNCH_CHECK();
+    }
+    grid.ensure(sizeof(T) * (size_t)G * nc * nbs);
+    for (int pass = 0; pass < 2; ++pass) {
+        cplx<T> *out = pass == 0 ? out_y : out_w;
+        if (!out) continue;
+        const TiledPoints<T> &P = pass == 0 ? pts : pts2;
+        SpreadArgs<T> A = spread_args(P);
+        A.y = pass == 0 ? y : nullptr;
+        A.grid = grid.as<T>();
+        A.grid_stride = G;
+        A.nrep = nbs;
+        A.counts = counts.as<int>();
+        A.counts_stride = n_;
+        launch_spread<T>(A, ntaps, VAL_COUNTS, nc * nbs, s);
+        run_fft(grid.as<T>(), out, nc * nbs, s);
+    }
+}
+
+template <typename T>
+void Plan<T>::backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s) {
+    require(spec.mode == CUNA_MODE_ACCURATE, "backward needs an accurate-mode plan");
+    require(g != nullptr, "gradient pointer is NULL");
+    if (!dy && !dx) return;
+    const int nc = ncurves();
+    const long N = (long)G / 2;
+    grid.ensure(sizeof(T) * (size_t)G * nc);
+    work.ensure(sizeof(cplx<T>) * (size_t)N * nc);
+    const double pi = 3.14159265358979323846;
+    LoadHermitian<T> ld;
+    ld.g = g;
+    ld.gstride = M;
+    ld.M = M;
+    ld.G = G;
+    ld.N = N;
+    ld.invG = T(1) / (T)G;
+    ld.coef = (T)(pi * pi * (double)beta);
+    ld.pref = (T)(1.0 / std::sqrt(pi * (double)beta));
+    ld.twG = TwiddleCache<T>::get().two_level(lgN + 1);
+    FftShape sh;
+    sh.lgN = lgN;
+    sh.batch = nc;
+    fft_c2r_inverse<T>(sh, ld, work.as<cplx<T>>(), grid.as<T>(), s);
+    GatherArgs<T> A{};
+    A.rec = pts.rec.template as<PointRec<T>>();
+    A.perm = pts.perm.template as<int>();
+    A.bucket = pts.bucket.template as<int>();
+    A.h = grid.as<T>();
+    A.h_stride = G;
+    A.y = y;
+    A.y_stride = 0;
+    A.dy = dy;
+    A.dx = dx;
+    A.out_stride = 0;
+    A.G = G;
+    A.tlog = pts.tlog;
+    A.ntiles = pts.ntiles;
+    A.Kc = pts.Kc;
+    A.olo = pts.olo;
+    A.nrep = 1;
+    A.bi = T(1) / beta;
+    A.wscale = wscale;
+    A.dxscale = (T)(2.0 * G / (double)beta);
+    for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
+    launch_gather<T>(A, ntaps, nc, s);
+}
+
+template <typename T>
+static int default_tlog() {
+    return sizeof(T) == 4 ? 11 : 10;
+}
+
+template <typename T>
+std::unique_ptr<Plan<T>> make_plan_accurate(const T *x, const double *t, double df,
+                                            const std::vector<int> &offsets, int M,
+                                            const ModeSpec &spec, hipStream_t s) {
+    require(M >= 2 && is_pow2(M), "M must be a power of two >= 2");
+    require(spec.sigma >= 2 && is_pow2(spec.sigma), "sigma must be a power of two >= 2");
+    require(spec.m >= 2 && spec.m <= kMaxTaps / 2, "m must be in [2, 16]");
+    const long Gl = (long)spec.sigma * M;
+    require(ilog2(Gl) - 1 <= max_fft_lg<T>(), "grid too large", CUNA_ERROR_UNSUPPORTED);
+    require(offsets.size() >= 2 && offsets.front() == 0, "offsets must start at 0");
+    for (size_t c = 1; c < offsets.size(); ++c)
+        require(offsets[c] >= offsets[c - 1], "offsets must be non-decreasing");
+    const long n = offsets.back();
+    require(n == 0 || x || t, "point array is NULL");
+    auto P = std::make_unique<Plan<T>>();
+    P->spec = spec;
+    P->M = M;
+    P->G = (int)Gl;
+    P->lgN = ilog2(Gl) - 1;
+    P->ntaps = 2 * spec.m;
+    P->olo = -(spec.m - 1);
+    const double pi = 3.14159265358979323846;
+    // beta = 2 sigma m / ((2 sigma - 1) pi): the reference's b with R = sigma
+    // (src/gaussian_filter.cu:129) and R_f = m
+    const double beta = 2.0 * spec.sigma * spec.m / ((2.0 * spec.sigma - 1.0) * pi);
+    P->beta = (T)beta;
+    P->wscale = T(1);
+    for (int q = 0; q < P->ntaps; ++q) {
+        double o = (double)(q + P->olo);
+        P->e3[q] = (T)std::exp(-o * o / (double)P->beta);
+    }
+    SetupArgs<T> A{};
+    A.x = x;
+    A.t = t;
+    A.df = df;
+    A.mode = t ? SETUP_ACC_T : SETUP_ACC_X;
+    int tl = spec.tile_log2 > 0 ? spec.tile_log2 : default_tlog<T>();
+    P->pts.build(A, offsets, P->G, tl, P->ntaps, true, false, s);
+    if (spec.flags & CUNA_PLAN_WINDOW2) {
+        // the points 2x: times with 2 df (FP64), or x doubled (exact in
+        // binary floating point)
+        SetupArgs<T> B = A;
+        if (t) {
+            B.df = 2.0 * df;
+            P->pts2.build(B, offsets, P->G, tl, P->ntaps, true, false, s);
+        } else {
+            DevBuf x2;
+            x2.alloc(sizeof(T) * std::max<long>(n, 1), s);
+            if (n > 0) {
+                hipLaunchKernelGGL(k_scale2<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, x,
+                                   x2.as<T>(), n);
+                CUNA_LAUNCH_CHECK();
+            }
+            B.x = x2.as<T>();
+            P->pts2.build(B, offsets, P->G, tl, P->ntaps, true, false, s);
+        }
+        P->has_pts2 = true;
+    }
+    return P;
+}
+
+template <typename T>
+void compat_constants(int n, int ng, T &b, T &normfac, int &R_f) {
+    const double pi = 3.14159265358979323846;
+    if (sizeof(T) == 4) {
+        // src/gaussian_filter.cu:125-134 in float:
+        // R = (float)ng / n; b = (float)(((2R)/(2R-1))_float * (6/PI)_double)
+        volatile float Rf = (float)ng / (float)n;
+        volatile float num = 2.0f * Rf;
+        volatile float den = 2.0f * Rf - 1.0f;
+        volatile float q = num / den;
+        b = (T)(float)((double)q * (6.0 / pi));
+        normfac = (T)(float)std::sqrt(2.0 * pi);
+        R_f = 6;
+    } else {
+        double Rd = (double)ng / n;
+        b = (T)(2.0 * Rd / (2.0 * Rd - 1.0) * (12.0 / pi));
+        normfac = (T)std::sqrt(2.0 * pi);
+        R_f = 12;
+    }
+}
+
+template <typename T>
+std::unique_ptr<Plan<T>> make_plan_compat(const T *x, int n, int ng, T *E1, T *E2, T *E3,
+                                          const ModeSpec &spec, hipStream_t s) {
+    require(n > 0, "n must be positive");
+    require(ng >= 4 && is_pow2(ng), "ng must be a power of two >= 4", CUNA_ERROR_UNSUPPORTED);
+    require(ilog2(ng) - 1 <= max_fft_lg<T>(), "ng too large", CUNA_ERROR_UNSUPPORTED);
+    // b = 2R/(2R-1) R_f/pi with R = ng/n is negative (NaN weights) for n >= 2 ng
+    require((long)n < 2L * ng, "n must be smaller than 2*ng (the filter width b is not positive)");
+    require(x != nullptr, "x is NULL");
+    auto P = std::make_unique<Plan<T>>();
+    P->spec = spec;
+    P->spec.mode = CUNA_MODE_COMPAT;
+    P->M = ng;
+    P->G = ng;
+    P->lgN = ilog2(ng) - 1;
+    compat_constants<T>(n, ng, P->b_c, P->normfac_c, P->R_f);
+    P->ntaps = 2 * P->R_f - 1;
+    P->olo = -(P->R_f - 1);
+    P->beta = P->b_c;
+    const double pi = 3.14159265358979323846;
+    // E1 carries 1/sqrt(pi b) (src/gaussian_filter.cu:219)
+    P->wscale = (T)(1.0 / std::sqrt(pi * (double)P->b_c));
+    for (int q = 0; q < P->ntaps; ++q) {
+        double o = (double)(q + P->olo);
+        P->e3[q] = (T)std::exp(-o * o / (double)P->b_c);
+    }
+    SetupArgs<T> A{};
+    A.x = x;
+    A.mode = SETUP_COMPAT;
+    A.R_f = P->R_f;
+    A.b = P->b_c;
+    A.e1s = P->wscale;
+    A.E1 = E1;
+    A.E2 = E2;
+    A.E3 = E3;
+    int tl = spec.tile_log2 > 0 ? spec.tile_log2 : default_tlog<T>();
+    std::vector<int> off = {0, n};
+    P->pts.build(A, off, ng, tl, P->ntaps, false, true, s);
+    return P;
+}
+
+}  // namespace cuna
+
+namespace cuna {
+
+// ------------------------------------------------------------- registry --
+// Associates the spread plan with the device filter_properties pointer that
+// generate_*filter_properties filled, so that the raw transforms find it.
+namespace {
+std::mutex g_reg_mu;
+std::unordered_map<const void *, std::shared_ptr<Plan<R>>> g_registry;
+}  // namespace
+
+static void reg_put(const void *key, std::unique_ptr<Plan<R>> p) {
+    std::shared_ptr<Plan<R>> sp(std::move(p));
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    g_registry[key] = sp;
+}
+static std::shared_ptr<Plan<R>> reg_get(const void *key) {
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    auto it = g_registry.find(key);
+    return it == g_registry.end() ? nullptr : it->second;
+}
+static void reg_drop(const void *key) {
+    std::shared_ptr<Plan<R>> victim;
+    {
+        std::lock_guard<std::mutex> lk(g_reg_mu);
+        auto it = g_registry.find(key);
+        if (it != g_registry.end()) {
+            victim = it->second;
+            g_registry.erase(it);
+        }
+    }
+}
+
+// the plan registered for (fprops, x, n, ng), or a fresh one
+static std::shared_ptr<Plan<R>> compat_plan_for(const R *x, int n, int ng,
+                                                const filter_properties *fp, hipStream_t s) {
+    auto p = fp ? reg_get(fp) : nullptr;
+    if (p && p->x_key == x && p->n() == n && p->G == ng) return p;
+    ModeSpec spec;
+    spec.mode = CUNA_MODE_COMPAT;
+    auto fresh = make_plan_compat<R>(x, n, ng, nullptr, nullptr, nullptr, spec, s);
+    fresh->x_key = x;
+    return std::shared_ptr<Plan<R>>(std::move(fresh));
+}
+
+static void generate_impl(const R *x, int n, int ng, filter_properties *h, filter_properties *d,
+                          hipStream_t s, bool sync) {
+    require(h != nullptr && d != nullptr, "filter_properties pointers are NULL");
+    require(n > 0 && ng > 0, "n and ng must be positive");
+    R b, nf;
+    int rf;
+    compat_constants<R>(n, ng, b, nf, rf);
+    h->b = b;
+    h->normfac = nf;
+    h->filter_radius = rf;
+    CUNA_HIP(hipMemcpyAsync(d, h, sizeof(filter_properties), hipMemcpyHostToDevice, s));
+    ModeSpec spec;
+    spec.mode = CUNA_MODE_COMPAT;
+    auto p = make_plan_compat<R>(x, n, ng, h->E1, h->E2, h->E3, spec, s);
+    p->x_key = x;
+    if (sync) CUNA_HIP(hipStreamSynchronize(s));
+    reg_put(d, std::move(p));
+}
+
+}  // namespace cuna
+
+using namespace cuna;
+
+// ------------------------------------------------------------------ errors --
+CUNA_API int cuna_get_last_error(void) { return g_err; }
+CUNA_API const char *cuna_get_last_error_string(void) { return g_msg.c_str(); }
+CUNA_API void cuna_clear_last_error(void) { clear_error(); }
+CUNA_API void cuna_set_abort_on_error(int enable) { g_abort = enable != 0; }
+CUNA_API int cuna_real_size(void) { return (int)sizeof(R); }
+CUNA_API int cuna_filter_radius(void) { return sizeof(R) == 4 ? 6 : 12; }
+
+// -------------------------------------------------- filter properties --
+CUNA_API void generate_filter_properties(const dTyp *x, const int n, const int ng,
+                                         filter_properties **fprops_host,
+                                         filter_properties **fprops_device) {
+    guarded([&] {
+        require(fprops_host && fprops_device, "output pointers are NULL");
+        require(n > 0 && ng > 0, "n and ng must be positive");
+        const int rf = cuna_filter_radius();
+        filter_properties *h = (filter_properties *)malloc(sizeof(filter_properties));
+        require(h != nullptr, "host allocation failed", CUNA_ERROR_OUT_OF_MEMORY);
+        memset(h, 0, sizeof *h);
+        filter_properties *d = nullptr;
+        try {
+            CUNA_HIP(hipMalloc(&d, sizeof(filter_properties)));
+            CUNA_HIP(hipMalloc(&h->E1, sizeof(R) * n));
+            CUNA_HIP(hipMalloc(&h->E2, sizeof(R) * n));
+            CUNA_HIP(hipMalloc(&h->E3, sizeof(R) * rf));
+            generate_impl(x, n, ng, h, d, nullptr, true);
+            CUNA_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            if (h->E1) (void)hipFree(h->E1);
+            if (h->E2) (void)hipFree(h->E2);
+            if (h->E3) (void)hipFree(h->E3);
+            if (d) (void)hipFree(d);
+            free(h);
+            throw;
+        }
+        *fprops_host = h;
+        *fprops_device = d;
+    });
+}
+
+CUNA_API void generate_pinned_filter_properties(const dTyp *x, const int n, const int ng,
+                                                filter_properties *h_fprops,
+                                                filter_properties *d_fprops, hipStream_t stream) {
+    guarded([&] {
+        require(h_fprops && h_fprops->E1 && h_fprops->E2 && h_fprops->E3,
+                "h_fprops and its E1/E2/E3 arrays must be allocated by the caller");
+        generate_impl(x, n, ng, h_fprops, d_fprops, stream, false);
+    });
+}
+
+CUNA_API void free_filter_properties(filter_properties *d_fp, filter_properties *fp) {
+    guarded([&] {
+        reg_drop(d_fp);
+        if (fp) {
+            if (fp->E1) CUNA_HIP(hipFree(fp->E1));
+            if (fp->E2) CUNA_HIP(hipFree(fp->E2));
+            if (fp->E3) CUNA_HIP(hipFree(fp->E3));
+        }
+        if (d_fp) CUNA_HIP(hipFree(d_fp));
+        free(fp);
+    });
+}
+
+CUNA_API void set_filter_properties(plan *p) {
+    guarded([&] {
+        require(p != nullptr, "plan is NULL");
+        p->filter_radius = cuna_filter_radius();
+        filter_properties *h = nullptr, *d = nullptr;
+        generate_filter_properties(p->g_x_data, p->Ndata, p->Ngrid, &h, &d);
+        if (g_err != CUNA_SUCCESS) fail(g_err, g_msg);
+        p->fprops_host = h;
+        p->fprops_device = d;
+    });
+}
+
+// --------------------------------------------------------- raw transforms --
+static void raw_impl(const dTyp *x, const dTyp *f_data, Complex *f_hat, int n, int ng,
+                     const filter_properties *fp, hipStream_t s) {
+    require(f_hat != nullptr, "f_hat is NULL");
+    auto p = compat_plan_for(x, n, ng, fp, s);
+    p->adjoint(f_data, reinterpret_cast<cplx<R> *>(f_hat), s);
+}
+
+CUNA_API void cunfft_adjoint_raw(const dTyp *x, const dTyp *f_data, Complex *f_hat, const int n,
+                                 const int ng, const filter_properties *gpu_fprops) {
+    guarded([&] { raw_impl(x, f_data, f_hat, n, ng, gpu_fprops, nullptr); });
+}
+
+CUNA_API void cunfft_adjoint_raw_async(const dTyp *x, const dTyp *f_data, Complex *f_hat,
+                                       const int n, const int ng,
+                                       const filter_properties *gpu_fprops, hipStream_t stream) {
+    guarded([&] { raw_impl(x, f_data, f_hat, n, ng, gpu_fprops, stream); });
+}
+
+CUNA_API void cunfft_adjoint_raw_async_bootstrap(const dTyp *x, const dTyp *f_data,
+                                                 Complex *f_hat, const int n, const int ng,
+                                                 const int nbs,
+                                                 const filter_properties *gpu_fprops,
+                                                 hipStream_t stream, const unsigned int seed) {
+    guarded([&] {
+        require(f_hat != nullptr, "f_hat is NULL");
+        require(nbs > 0, "nbs must be positive");
+        auto p = compat_plan_for(x, n, ng, gpu_fprops, stream);
+        p->bootstrap_compat(f_data, reinterpret_cast<cplx<R> *>(f_hat), nbs, seed, stream);
+    });
+}
+
+// ------------------------------------------------------------ plan path --
+CUNA_API void init_plan(plan *p, const dTyp *x, const dTyp *f, int Ndata, int Ngrid,
+                        unsigned int plan_flags) {
+    guarded([&] {
+        require(p != nullptr, "plan is NULL");
+        require(x != nullptr, "x is NULL");
+        require(Ndata > 0 && Ngrid > 0, "Ndata and Ngrid must be positive");
+        memset(p, 0, sizeof *p);
+        p->Ndata = Ndata;
+        p->Ngrid = Ngrid;
+        p->flags = plan_flags;
+        p->x_data = (R *)malloc(sizeof(R) * Ndata);
+        p->f_data = (R *)malloc(sizeof(R) * Ndata);
+        p->f_hat = (Complex *)malloc(sizeof(Complex) * Ngrid);
+        if (plan_flags & CALCULATE_WINDOW_FUNCTION)
+            p->f_hat_win = (Complex *)malloc(sizeof(Complex) * Ngrid);
+        require(p->x_data && p->f_data && p->f_hat, "host allocation failed",
+                CUNA_ERROR_OUT_OF_MEMORY);
+        memcpy(p->x_data, x, sizeof(R) * Ndata);
+        // src/utils.cu:112-116 leaves f_data uninitialised for f == NULL;
+        // here NULL means unit weights
+        if (f)
+            memcpy(p->f_data, f, sizeof(R) * Ndata);
+        else
+            for (int i = 0; i < Ndata; ++i) p->f_data[i] = R(1);
+        CUNA_HIP(hipMalloc(&p->g_f_data, sizeof(R) * Ndata));
+        CUNA_HIP(hipMalloc(&p->g_x_data, sizeof(R) * Ndata));
+        CUNA_HIP(hipMalloc(&p->g_f_hat, sizeof(Complex) * Ngrid));
+        CUNA_HIP(hipMemset(p->g_f_hat, 0, sizeof(Complex) * Ngrid));
+        if (plan_flags & CALCULATE_WINDOW_FUNCTION) {
+            CUNA_HIP(hipMalloc(&p->g_f_hat_win, sizeof(Complex) * Ngrid));
+            CUNA_HIP(hipMemset(p->g_f_hat_win, 0, sizeof(Complex) * Ngrid));
+        }
+        CUNA_HIP(hipMemcpy(p->g_f_data, p->f_data, sizeof(R) * Ndata, hipMemcpyHostToDevice));
+        CUNA_HIP(hipMemcpy(p->g_x_data, p->x_data, sizeof(R) * Ndata, hipMemcpyHostToDevice));
+        CUNA_HIP(hipDeviceSynchronize());
+        set_filter_properties(p);
+        if (g_err != CUNA_SUCCESS) fail(g_err, g_msg);
+    });
+}
+
+CUNA_API void cunfft_adjoint_from_plan(plan *p) {
+    guarded([&] {
+        require(p != nullptr && p->g_x_data && p->g_f_hat, "plan is not initialised");
+        raw_impl(p->g_x_data, p->g_f_data, p->g_f_hat, p->Ndata, p->Ngrid, p->fprops_device,
+                 nullptr);
+        if (p->flags & CALCULATE_WINDOW_FUNCTION)
+            raw_impl(p->g_x_data, nullptr, p->g_f_hat_win, p->Ndata, p->Ngrid, p->fprops_device,
+                     nullptr);
+        // the reference meant to copy back unless DONT_TRANSFER_TO_CPU
+        // (src/cuna_kernel.cu:296; its test `!(flags | ...)` is always false)
+        if (!(p->flags & DONT_TRANSFER_TO_CPU)) {
+            CUNA_HIP(hipMemcpy(p->f_hat, p->g_f_hat, sizeof(Complex) * p->Ngrid,
+                               hipMemcpyDeviceToHost));
+            if ((p->flags & CALCULATE_WINDOW_FUNCTION) && p->f_hat_win)
+                CUNA_HIP(hipMemcpy(p->f_hat_win, p->g_f_hat_win, sizeof(Complex) * p->Ngrid,
+                                   hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+CUNA_API void free_plan(plan *p) {
+    guarded([&] {
+        if (!p) return;
+        free(p->f_hat);
+        free(p->f_hat_win);
+        free(p->x_data);
+        free(p->f_data);
+        if (p->fprops_device || p->fprops_host) free_filter_properties(p->fprops_device, p->fprops_host);
+        if (p->g_f_hat) CUNA_HIP(hipFree(p->g_f_hat));
+        if (p->g_f_hat_win) CUNA_HIP(hipFree(p->g_f_hat_win));
+        if (p->g_f_data) CUNA_HIP(hipFree(p->g_f_data));
+        if (p->g_x_data) CUNA_HIP(hipFree(p->g_x_data));
+        free(p);
+    });
+}
+
+CUNA_API void cunfft_adjoint(const dTyp *x, const dTyp *f, cTyp *fft, int n, int ng,
+                             unsigned int flags) {
+    guarded([&] {
+        require(fft != nullptr, "fft is NULL");
+        plan *p = (plan *)malloc(sizeof(plan));
+        require(p != nullptr, "host allocation failed", CUNA_ERROR_OUT_OF_MEMORY);
+        memset(p, 0, sizeof *p);
+        init_plan(p, x, f, n, ng, flags & ~DONT_TRANSFER_TO_CPU);
+        int err = g_err;
+        std::string msg = g_msg;
+        if (err == CUNA_SUCCESS) {
+            cunfft_adjoint_from_plan(p);
+            err = g_err;
+            msg = g_msg;
+        }
+        if (err == CUNA_SUCCESS) memcpy((void *)fft, p->f_hat, sizeof(Complex) * ng);
+        free_plan(p);
+        if (err != CUNA_SUCCESS) fail(err, msg);
+    });
+}
+
+// ------------------------------------------------------------- extensions --
+struct cuna_nfft_plan_s {
+    std::unique_ptr<Plan<R>> p;
+};
+
+static ModeSpec spec_from(const cuna_nfft_params *params) {
+    cuna_nfft_params d;
+    cuna_nfft_default_params(&d);
+    const cuna_nfft_params &q = params ? *params : d;
+    ModeSpec s;
+    s.mode = q.mode;
+    s.sigma = q.sigma;
+    s.m = q.m;
+    s.tile_log2 = q.tile_log2;
+    s.flags = q.flags;
+    require(s.mode == CUNA_MODE_ACCURATE || s.mode == CUNA_MODE_COMPAT, "unknown mode");
+    require(s.tile_log2 == 0 || (s.tile_log2 >= 6 && s.tile_log2 <= 14), "tile_log2 out of range");
+    return s;
+}
+
+CUNA_API void cuna_nfft_default_params(cuna_nfft_params *p) {
+    if (!p) return;
+    p->mode = CUNA_MODE_ACCURATE;
+    p->sigma = 2;
+    p->m = sizeof(R) == 4 ? 8 : 13;
+    p->tile_log2 = 0;
+    p->flags = 0;
+}
+
+CUNA_API int cuna_nfft_plan_create(cuna_nfft_plan *plan, const dTyp *x, int n, int M,
+                                   const cuna_nfft_params *params, hipStream_t stream) {
+    return guarded([&] {
+        require(plan != nullptr, "plan is NULL");
+        require(n >= 0, "n must be non-negative");
+        ModeSpec spec = spec_from(params);
+        auto h = new cuna_nfft_plan_s;
+        try {
+            if (spec.mode == CUNA_MODE_COMPAT)
+                h->p = make_plan_compat<R>(x, n, M, nullptr, nullptr, nullptr, spec, stream);
+            else
+                h->p = make_plan_accurate<R>(x, nullptr, 0.0, {0, n}, M, spec, stream);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *plan = h;
+    });
+}
+
+CUNA_API int cuna_nfft_plan_create_times(cuna_nfft_plan *plan, const double *t, int n, double df,
+                                         int M, const cuna_nfft_params *params,
+                                         hipStream_t stream) {
+    return guarded([&] {
+        require(plan != nullptr, "plan is NULL");
+        require(n >= 0, "n must be non-negative");
+        require(std::isfinite(df) && df > 0, "df must be positive");
+        ModeSpec spec = spec_from(params);
+        require(spec.mode == CUNA_MODE_ACCURATE, "times plans are accurate-mode plans");
+        require(n == 0 || t != nullptr, "t is NULL");
+        auto h = new cuna_nfft_plan_s;
+        try {
+            h->p = make_plan_accurate<R>(nullptr, t, df, {0, n}, M, spec, stream);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *plan = h;
+    });
+}
+
+CUNA_API int cuna_nfft_plan_create_batch(cuna_nfft_plan *plan, const dTyp *x, const int *offsets,
+                                         int ncurves, int M, const cuna_nfft_params *params,
+                                         hipStream_t stream) {
+    return guarded([&] {
+        require(plan != nullptr && offsets != nullptr, "plan or offsets is NULL");
+        require(ncurves >= 1, "ncurves must be positive");
+        ModeSpec spec = spec_from(params);
+        require(spec.mode == CUNA_MODE_ACCURATE, "batched plans are accurate-mode plans");
+        std::vector<int> off(offsets, offsets + ncurves + 1);
+        auto h = new cuna_nfft_plan_s;
+        try {
+            h->p = make_plan_accurate<R>(x, nullptr, 0.0, off, M, spec, stream);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *plan = h;
+    });
+}
+
+CUNA_API int cuna_nfft_adjoint(cuna_nfft_plan plan, const dTyp *y, Complex *out,
+                               hipStream_t stream) {
+    return guarded([&] {
+        require(plan && plan->p, "plan is NULL");
+        plan->p->adjoint(y, reinterpret_cast<cplx<R> *>(out), stream);
+    });
+}
+
+CUNA_API int cuna_nfft_bootstrap(cuna_nfft_plan plan, const dTyp *y, int nbs, unsigned int seed,
+                                 Complex *out_y, Complex *out_w, hipStream_t stream) {
+    return guarded([&] {
+        require(plan && plan->p, "plan is NULL");
+        plan->p->bootstrap(y, nbs, seed, reinterpret_cast<cplx<R> *>(out_y),
+                           reinterpret_cast<cplx<R> *>(out_w), stream);
+    });
+}
+
+CUNA_API int cuna_nfft_backward(cuna_nfft_plan plan, const dTyp *y, const Complex *grad_out,
+                                dTyp *grad_y, dTyp *grad_x, hipStream_t stream) {
+    return guarded([&] {
+        require(plan && plan->p, "plan is NULL");
+        plan->p->backward(y, reinterpret_cast<const cplx<R> *>(grad_out), grad_y, grad_x, stream);
+    });
+}
+
+CUNA_API int cuna_nfft_plan_info(cuna_nfft_plan plan, cuna_nfft_info *info) {
+    return guarded([&] {
+        require(plan && plan->p && info, "plan or info is NULL");
+        const Plan<R> &p = *plan->p;
+        info->mode = p.spec.mode;
+        info->sigma = p.spec.mode == CUNA_MODE_ACCURATE ? p.spec.sigma : 0;
+        info->m = p.spec.mode == CUNA_MODE_ACCURATE ? p.spec.m : p.R_f;
+        info->ntaps = p.ntaps;
+        info->n = (int)p.n();
+        info->ncurves = p.ncurves();
+        info->M = p.M;
+        info->G = p.G;
+        info->tile_log2 = p.pts.tlog;
+        info->ntiles = p.pts.ntiles;
+        info->beta = (double)p.beta;
+    });
+}
+
+CUNA_API int cuna_nfft_plan_destroy(cuna_nfft_plan plan) {
+    return guarded([&] { delete plan; });
+}
+
+CUNA_API int cuna_fft(Complex *data, int n, int batch, int sign, hipStream_t stream) {
+    return guarded([&] {
+        require(data != nullptr, "data is NULL");
+        require(n >= 2 && is_pow2(n), "n must be a power of two >= 2", CUNA_ERROR_UNSUPPORTED);
+        require(batch >= 1, "batch must be positive");
+        FftShape sh;
+        sh.lgN = ilog2(n);
+        sh.batch = batch;
+        DevBuf work;
+        if (sh.lgN > max_small_lg<R>()) work.alloc(sizeof(cplx<R>) * (size_t)n * batch, stream);
+        fft_complex<R>(sh, reinterpret_cast<cplx<R> *>(data), work.as<cplx<R>>(), sign, stream);
+    });
+}

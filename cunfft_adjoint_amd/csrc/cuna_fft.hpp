@@ -1,0 +1,482 @@
+// cuna_fft.hpp -- hand-written Stockham FFT for gfx950, staged in LDS.
+//
+// Replaces the cuFFT stage of the reference (cufftExec{C2C,Z2Z} with
+// CUFFT_INVERSE, unnormalised, in place: src/cuna_kernel.cu:43-49,80-99,
+// 321-333,360-374,411-430).  The grid of the adjoint NFFT is real, so a real
+// grid of G cells is transformed as a complex sequence of N = G/2 values
+// z[n] = grid[2n] + i grid[2n+1] (the same memory), and the spectrum is
+// recovered pairwise from Z[r], Z[N-r]; the window deconvolution is applied
+// in that same epilogue ("fused into the last pass").
+//
+// Sizes: N <= 2^14 (float) / 2^13 (double) run as one workgroup per
+// transform entirely in LDS; larger N = L1*L2 run as two passes (column
+// FFTs of length L2 with twiddles, then row FFTs of length L1 whose rows are
+// processed in mirror pairs so that the real post-processing stays local).
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "cuna_common.hpp"
+
+namespace cuna {
+
+extern __shared__ __attribute__((aligned(16))) unsigned char cuna_smem[];
+
+// ---------------------------------------------------------- butterflies --
+// In-register DFTs of radix 2/4/8/16, v[n] -> v[k] = sum_n v[n] w_R^{S n k}.
+template <typename T>
+__device__ __forceinline__ cplx<T> rot(cplx<T> z, T c, T s) {  // z * (c + i s)
+    return cmk(z.x * c - z.y * s, z.x * s + z.y * c);
+}
+template <typename T, int S>
+__device__ __forceinline__ cplx<T> mul_si(cplx<T> z) {  // z * (S i)
+    return S > 0 ? cmk(-z.y, z.x) : cmk(z.y, -z.x);
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void dft2(cplx<T> *v) {
+    cplx<T> t = v[0] - v[1];
+    v[0] = v[0] + v[1];
+    v[1] = t;
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void dft4(cplx<T> &a0, cplx<T> &a1, cplx<T> &a2, cplx<T> &a3) {
+    cplx<T> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = mul_si<T, S>(a1 - a3);
+    a0 = t0 + t2;
+    a2 = t0 - t2;
+    a1 = t1 + t3;
+    a3 = t1 - t3;
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void dft8(cplx<T> *v) {
+    const T h = T(0.70710678118654752440);
+    cplx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cplx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<T, S>(e0, e1, e2, e3);
+    dft4<T, S>(o0, o1, o2, o3);
+    o1 = rot(o1, h, S * h);
+    o2 = mul_si<T, S>(o2);
+    o3 = rot(o3, -h, S * h);
+    v[0] = e0 + o0;
+    v[4] = e0 - o0;
+    v[1] = e1 + o1;
+    v[5] = e1 - o1;
+    v[2] = e2 + o2;
+    v[6] = e2 - o2;
+    v[3] = e3 + o3;
+    v[7] = e3 - o3;
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void dft16(cplx<T> *v) {
+    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);
+    const T h = T(0.70710678118654752440);
+    cplx<T> e[8], o[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        e[n] = v[2 * n];
+        o[n] = v[2 * n + 1];
+    }
+    dft8<T, S>(e);
+    dft8<T, S>(o);
+    o[1] = rot(o[1], c1, S * s1);
+    o[2] = rot(o[2], h, S * h);
+    o[3] = rot(o[3], s1, S * c1);
+    o[4] = mul_si<T, S>(o[4]);
+    o[5] = rot(o[5], -s1, S * c1);
+    o[6] = rot(o[6], -h, S * h);
+    o[7] = rot(o[7], -c1, S * s1);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        v[k] = e[k] + o[k];
+        v[k + 8] = e[k] - o[k];
+    }
+}
+
+template <typename T, int R, int S>
+__device__ __forceinline__ void dft(cplx<T> *v) {
+    if constexpr (R == 2) dft2<T, S>(v);
+    if constexpr (R == 4) dft4<T, S>(v[0], v[1], v[2], v[3]);
+    if constexpr (R == 8) dft8<T, S>(v);
+    if constexpr (R == 16) dft16<T, S>(v);
+}
+
+// ------------------------------------------------------------ radix plan --
+// log2 of the radix of pass p for a length-2^lg transform: radix 16 passes,
+// with the remainder taken by one radix-8/4 pass (or 8+4 for lg = 4k+1).
+constexpr int num_passes(int lg) {
+    if (lg <= 0) return 0;
+    if (lg == 1) return 1;
+    int n16 = lg / 4, rem = lg % 4;
+    if (rem == 0) return n16;
+    if (rem == 1) return n16 + 1;  // (n16-1) x16, then 8, 4
+    return n16 + 1;
+}
+constexpr int pass_log2(int lg, int p) {
+    if (lg == 1) return 1;
+    int n16 = lg / 4, rem = lg % 4;
+    if (rem == 0) return 4;
+    if (rem == 1) {
+        if (p < n16 - 1) return 4;
+        return p == n16 - 1 ? 3 : 2;
+    }
+    return p < n16 ? 4 : rem;
+}
+
+// ----------------------------------------------------------- LDS Stockham --
+// W interleaved sequences of length L = 2^LG: element e of sequence w lives at
+// lds[e*W + w].  Pass 0 reads its inputs through `ld(e, w)`; every pass writes
+// LDS; the routine returns with natural-order results in LDS after a barrier.
+// tw[e] = exp(+2 pi i e / L) (conjugated for S = -1).
+template <typename T, int S, int LG, int W, int NT, int P, int LOGNS, class Loader>
+__device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict__ tw,
+                                         const Loader &ld) {
+    constexpr int NP = num_passes(LG);
+    if constexpr (P < NP) {
+        constexpr int LR = pass_log2(LG, P);
+        constexpr int R = 1 << LR;
+        constexpr int L = 1 << LG;
+        constexpr int Q = L >> LR;  // butterflies per sequence
+        constexpr int NB = W * Q;
+        constexpr int BPT = (NB + NT - 1) / NT;
+        constexpr bool EXACT = (NB % NT) == 0;
+        cplx<T> v[BPT][R];
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < BPT; ++q) {
+            const int g = tid + q * NT;
+            if (EXACT || g < NB) {
+                const int w = g % W, j = g / W;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = j + r * Q;
+                    if constexpr (P == 0)
+                        v[q][r] = ld(e, w);
+                    else
+                        v[q][r] = lds[e * W + w];
+                }
+            }
+        }
+        if constexpr (P > 0) __syncthreads();
+#pragma unroll
+        for (int q = 0; q < BPT; ++q) {
+            const int g = tid + q * NT;
+            if (EXACT || g < NB) {
+                const int w = g % W, j = g / W;
+                const int k = j & ((1 << LOGNS) - 1);
+                if constexpr (P > 0) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        cplx<T> t = tw[(k * r) << (LG - LOGNS - LR)];
+                        if constexpr (S < 0) t.y = -t.y;
+                        v[q][r] = v[q][r] * t;
+                    }
+                }
+                dft<T, R, S>(v[q]);
+                const int base = ((j >> LOGNS) << (LOGNS + LR)) + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[(base + (r << LOGNS)) * W + w] = v[q][r];
+            }
+        }
+        __syncthreads();
+        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader>(lds, tw, ld);
+    }
+}
+
+template <typename T, int S, int LG, int W, int NT, class Loader>
+__device__ __forceinline__ void lds_fft(cplx<T> *lds, const cplx<T> *__restrict__ tw,
+                                        const Loader &ld) {
+    fft_pass<T, S, LG, W, NT, 0, 0, Loader>(lds, tw, ld);
+}
+
+// ------------------------------------------------------ twiddle tables --
+// Two-level table for exp(+2 pi i e / Nfull), e < Nfull:
+//   lo[e & (B-1)] * hi[e >> log2 B], lo[q] = w^q, hi[q] = w^(q B).
+template <typename T>
+struct Tw2 {
+    const cplx<T> *lo;
+    const cplx<T> *hi;
+    int lb;  // log2 B
+    __device__ __forceinline__ cplx<T> operator()(long e) const {
+        return lo[e & ((1L << lb) - 1)] * hi[e >> lb];
+    }
+};
+
+// Device twiddle tables, generated in long double on the host once per size
+// and kept for the life of the process (a few KB each).
+template <typename T>
+class TwiddleCache {
+  public:
+    static TwiddleCache &get() {
+        static TwiddleCache c;
+        return c;
+    }
+    // exp(+2 pi i e / L) for e in [0, L)
+    const cplx<T> *fine(int lg) { return table(lg, 0, 1L << lg); }
+    Tw2<T> two_level(int lg) {
+        int lb = (lg + 1) / 2;
+        Tw2<T> t;
+        t.lo = table(lg, 1, 1L << lb);          // w_N^q, q < B
+        t.hi = table(lg - lb, 0, 1L << (lg - lb));  // w_{N/B}^q
+        t.lb = lb;
+        return t;
+    }
+
+  private:
+    std::mutex mu_;
+    std::map<std::tuple<int, int, int>, void *> tabs_;
+    const cplx<T> *table(int lg, int kind, long count) {
+        int dev = 0;
+        CUNA_HIP(hipGetDevice(&dev));
+        std::lock_guard<std::mutex> lk(mu_);
+        auto key = std::make_tuple(dev, lg, kind);
+        auto it = tabs_.find(key);
+        if (it != tabs_.end()) return static_cast<const cplx<T> *>(it->second);
+        std::vector<cplx<T>> h(count);
+        const long double two_pi = 6.283185307179586476925286766559005768L;
+        long N = 1L << lg;
+        for (long e = 0; e < count; ++e) {
+            long double a = two_pi * (long double)e / (long double)N;
+            h[e].x = (T)cosl(a);
+            h[e].y = (T)sinl(a);
+        }
+        void *d = nullptr;
+        CUNA_HIP(hipMalloc(&d, sizeof(cplx<T>) * count));
+        CUNA_HIP(hipMemcpy(d, h.data(), sizeof(cplx<T>) * count, hipMemcpyHostToDevice));
+        tabs_[key] = d;
+        return static_cast<const cplx<T> *>(d);
+    }
+};
+
+// ------------------------------------------------------------- sinks --
+// A sink receives one spectral value A[f] of the real grid (f in [0, N]) and
+// writes it, and its conjugate image A[G-f], where the output wants them.
+
+// Accurate mode: out[M/2 + k], k = -M/2 .. M/2-1, deconvolved by
+// D(k) = exp(pi^2 beta k^2/G^2)/sqrt(pi beta).
+template <typename T>
+struct SinkAccurate {
+    cplx<T> *out;
+    long M;
+    long ostride;  // per batch
+    T invG, coef, pref;
+    __device__ __forceinline__ void operator()(cplx<T> *o, long f, cplx<T> A) const {
+        if (f <= M / 2) {
+            T kk = (T)f * invG;
+            T D = pref * dexp(coef * kk * kk);
+            if (f < M / 2) o[M / 2 + f] = A * D;
+            if (f > 0) o[M / 2 - f] = conjg(A) * D;
+        }
+    }
+};
+
+// Compat mode (src/gaussian_filter.cu:278-289): out[r] = A[r] * normfac *
+// exp((r/ng)^2 * b/4) for every r in [0, ng).
+template <typename T>
+struct SinkCompat {
+    cplx<T> *out;
+    long G;
+    long ostride;
+    T invG, qb, normfac;  // qb = b/4
+    __device__ __forceinline__ T fac(long r) const {
+        T K = (T)r * invG;
+        return normfac * dexp(K * K * qb);
+    }
+    __device__ __forceinline__ void operator()(cplx<T> *o, long f, cplx<T> A) const {
+        o[f] = A * fac(f);
+        if (f != 0 && 2 * f != G) o[G - f] = conjg(A) * fac(G - f);
+    }
+};
+
+// A[r] and A[N-r] of the real grid from Z[r], Z[N-r]:
+//   E = (Z[r] + conj Z[N-r])/2, O = (Z[r] - conj Z[N-r])/(2i),
+//   A[r] = E + w_G^r O,  A[N-r] = conj(E - w_G^r O).
+template <typename T, class Sink>
+__device__ __forceinline__ void emit_pair(const Sink &sk, cplx<T> *o, long r, long N, cplx<T> Zr,
+                                          cplx<T> Znr, cplx<T> wr) {
+    cplx<T> c = conjg(Znr);
+    cplx<T> E = (Zr + c) * T(0.5);
+    cplx<T> D = (Zr - c) * T(0.5);
+    cplx<T> O = cmk(D.y, -D.x);
+    cplx<T> wO = wr * O;
+    sk(o, r, E + wO);
+    if (2 * r != N) sk(o, N - r, conjg(E - wO));
+}
+
+// ------------------------------------------------------------- loaders --
+// Forward: the real grid read as complex z[n] (batch stride N).
+template <typename T>
+struct LoadGrid {
+    const cplx<T> *z;
+    long stride;  // per batch
+    __device__ __forceinline__ void advance(long b) { z += b * stride; }
+    __device__ __forceinline__ cplx<T> at(long n) const { return z[n]; }
+};
+
+// Backward (C2R of a Hermitian spectrum): builds Z[q] = E_q + i O_q with
+// E_q = H_q + H_{q+N}, O_q = w_G^{-q} (H_q - H_{q+N}),
+// H_f = (c_f + conj c_{G-f})/2, c_f = D(k) g[M/2 + k] for f = k mod G in band.
+template <typename T>
+struct LoadHermitian {
+    const cplx<T> *g;  // M values, natural order k = -M/2..M/2-1
+    long gstride;      // per batch
+    long M, G, N;
+    T invG, coef, pref;
+    Tw2<T> twG;
+    __device__ __forceinline__ void advance(long b) { g += b * gstride; }
+    __device__ __forceinline__ cplx<T> c(long f) const {
+        long k;
+        if (f < M / 2)
+            k = f;
+        else if (f >= G - M / 2 && f < G)
+            k = f - G;
+        else
+            return cmk(T(0), T(0));
+        T kk = (T)k * invG;
+        T D = pref * dexp(coef * kk * kk);
+        return g[M / 2 + k] * D;
+    }
+    __device__ __forceinline__ cplx<T> H(long f) const {
+        long mf = f == 0 ? 0 : G - f;
+        return (c(f) + conjg(c(mf))) * T(0.5);
+    }
+    __device__ __forceinline__ cplx<T> at(long q) const {
+        cplx<T> h0 = H(q), h1 = H(q + N);
+        cplx<T> E = h0 + h1;
+        cplx<T> w = conjg(twG(q));
+        cplx<T> O = w * (h0 - h1);
+        return E + mul_i(O);
+    }
+};
+
+// ------------------------------------------------------------- kernels --
+// One workgroup per transform (batch = blockIdx.x), whole sequence in LDS.
+// MODE 0: real post-processing into a sink; MODE 1: plain store of z.
+template <typename T, int S, int LG, int NT, class Loader, class Sink, int MODE>
+__global__ __launch_bounds__(NT) void k_fft_small(Loader ld, Sink sk, cplx<T> *plain_out,
+                                                  const cplx<T> *tw, Tw2<T> twG) {
+    constexpr long N = 1L << LG;
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
+    const long b = blockIdx.x;
+    Loader lb = ld;
+    lb.advance(b);
+    auto fetch = [&](int e, int) { return lb.at(e); };
+    lds_fft<T, S, LG, 1, NT>(lds, tw, fetch);
+    if constexpr (MODE == 0) {
+        cplx<T> *o = sk.out + b * sk.ostride;
+        for (long r = threadIdx.x; r <= N / 2; r += NT) {
+            cplx<T> Zr = lds[r];
+            cplx<T> Zn = lds[(N - r) & (N - 1)];
+            emit_pair(sk, o, r, N, Zr, Zn, twG(r));
+        }
+    } else {
+        cplx<T> *o = plain_out + b * N;
+        for (long e = threadIdx.x; e < N; e += NT) o[e] = lds[e];
+    }
+}
+
+// Pass 1 of the two-pass FFT: W consecutive columns n1 of the L1 x L2 view
+// (n = n1 + L1 n2) per workgroup; length-L2 FFT over n2, multiply by
+// w_N^(S n1 r2), store at n1 + L1 r2 of dst (may alias the source).
+template <typename T, int S, int LG2, int W, int NT, class Loader>
+__global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long dst_stride, int L1,
+                                                const cplx<T> *tw, Tw2<T> twN) {
+    constexpr int L2 = 1 << LG2;
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
+    const int nblk = L1 / W;
+    const int blk = xcd_remap(blockIdx.x, nblk);
+    const long b = blockIdx.y;
+    const int n10 = blk * W;
+    Loader lb = ld;
+    lb.advance(b);
+    auto fetch = [&](int e, int w) { return lb.at(n10 + w + (long)L1 * e); };
+    lds_fft<T, S, LG2, W, NT>(lds, tw, fetch);
+    cplx<T> *o = dst + b * dst_stride;
+    for (int idx = threadIdx.x; idx < W * L2; idx += NT) {
+        const int w = idx % W, r2 = idx / W;
+        cplx<T> t = twN((long)(n10 + w) * r2);
+        if constexpr (S < 0) t.y = -t.y;
+        o[n10 + w + (long)L1 * r2] = lds[r2 * W + w] * t;
+    }
+}
+
+// Pass 2, real post-processing: workgroup i holds rows rho = i*H + h and their
+// mirrors (L2 - rho) mod L2 (row 0's mirror slot holds row L2/2; both are
+// self-paired).  Length-L1 FFTs over n1, then pairs (Z[r], Z[N-r]) with
+// r = rho + L2 r1 go through emit_pair into the sink.
+template <typename T, int LG1, int H, int NT, class Sink>
+__global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long src_stride,
+                                                     int L2, Sink sk, const cplx<T> *tw,
+                                                     Tw2<T> twG) {
+    constexpr int L1 = 1 << LG1;
+    constexpr int W = 2 * H;
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
+    const int nblk = L2 / (2 * H);
+    const int i = xcd_remap(blockIdx.x, nblk);
+    const long b = blockIdx.y;
+    const long N = (long)L1 * L2;
+    const cplx<T> *z = src + b * src_stride;
+    auto row_of = [&](int s) -> int {
+        int h = s % H;
+        int rho = i * H + h;
+        if (s < H) return rho;
+        return rho == 0 ? L2 / 2 : L2 - rho;
+    };
+    auto fetch = [&](int e, int s) { return z[(long)row_of(s) * L1 + e]; };
+    lds_fft<T, 1, LG1, W, NT>(lds, tw, fetch);
+    cplx<T> *o = sk.out + b * sk.ostride;
+    // pairs: slot h, element r1 of the primary row with its mirror element
+    for (int idx = threadIdx.x; idx < H * L1; idx += NT) {
+        const int h = idx % H, r1 = idx / H;
+        const int rho = i * H + h;
+        if (rho != 0) {
+            long r = rho + (long)L2 * r1;
+            cplx<T> Zr = lds[r1 * W + h];
+            cplx<T> Zn = lds[(L1 - 1 - r1) * W + (H + h)];
+            emit_pair(sk, o, r, N, Zr, Zn, twG(r));
+        } else {
+            // row 0 (slot h=0): element r1 pairs with (L1 - r1) mod L1, r1 <= L1/2
+            if (r1 <= L1 / 2) {
+                long r = (long)L2 * r1;
+                cplx<T> Zr = lds[r1 * W + 0];
+                cplx<T> Zn = lds[((L1 - r1) & (L1 - 1)) * W + 0];
+                emit_pair(sk, o, r, N, Zr, Zn, twG(r));
+            }
+            // row L2/2 (slot H): element r1 pairs with L1-1-r1, r1 < L1/2
+            if (r1 < L1 / 2) {
+                long r = L2 / 2 + (long)L2 * r1;
+                cplx<T> Zr = lds[r1 * W + H];
+                cplx<T> Zn = lds[(L1 - 1 - r1) * W + H];
+                emit_pair(sk, o, r, N, Zr, Zn, twG(r));
+            }
+        }
+    }
+}
+
+// Pass 2, plain: H consecutive rows per workgroup, length-L1 FFT, result
+// X[r2 + L2 r1] stored in natural order.
+template <typename T, int S, int LG1, int H, int NT>
+__global__ __launch_bounds__(NT) void k_fft_row_plain(const cplx<T> *src, long src_stride,
+                                                      int L2, cplx<T> *dst, long dst_stride,
+                                                      const cplx<T> *tw) {
+    constexpr int L1 = 1 << LG1;
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
+    const int nblk = L2 / H;
+    const int i = xcd_remap(blockIdx.x, nblk);
+    const long b = blockIdx.y;
+    const cplx<T> *z = src + b * src_stride;
+    auto fetch = [&](int e, int h) { return z[(long)(i * H + h) * L1 + e]; };
+    lds_fft<T, S, LG1, H, NT>(lds, tw, fetch);
+    cplx<T> *o = dst + b * dst_stride;
+    for (int idx = threadIdx.x; idx < H * L1; idx += NT) {
+        const int h = idx % H, r1 = idx / H;
+        o[(i * H + h) + (long)L2 * r1] = lds[r1 * H + h];
+    }
+}
+
+}  // namespace cuna

@@ -1,0 +1,323 @@
+// cuna_kernels.hip -- instantiation and launch of the FFT and spread kernels
+// for the library's real type (float in libcunaf.so, double in libcunad.so).
+#include "cuna_launch.hpp"
+
+namespace cuna {
+
+template <typename K>
+static void set_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        CUNA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+static constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ----------------------------------------------------------- FFT config --
+// threads per workgroup for W sequences of length 2^LG
+constexpr int fft_threads(int lg, int w) {
+    return clampi(((1 << lg) * w) / 16, 64, 1024);
+}
+// sequences per workgroup so that the LDS tile stays at 64 KB
+template <typename T>
+constexpr int seq_per_wg(int lg, int cap) {
+    return clampi((int)(65536 / ((1L << lg) * (long)sizeof(cplx<T>))), 1, cap);
+}
+// two-pass split N = L1 * L2, L2 >= L1
+constexpr int split_lg1(int lgN) { return lgN / 2; }
+
+template <typename T, int S, int LG, class Loader, class Sink, int MODE>
+static void run_small(const FftShape &sh, const Loader &ld, const Sink &sk, cplx<T> *plain_out,
+                      hipStream_t s) {
+    constexpr int NT = fft_threads(LG, 1);
+    auto &tc = TwiddleCache<T>::get();
+    const cplx<T> *tw = tc.fine(LG);
+    Tw2<T> twG = tc.two_level(LG + 1);
+    size_t lds = sizeof(cplx<T>) << LG;
+    auto k = k_fft_small<T, S, LG, NT, Loader, Sink, MODE>;
+    set_lds(k, lds);
+    hipLaunchKernelGGL(k, dim3(sh.batch), dim3(NT), lds, s, ld, sk, plain_out, tw, twG);
+    CUNA_LAUNCH_CHECK();
+}
+
+template <typename T, int S, int LG2, class Loader>
+static void run_col(const FftShape &sh, const Loader &ld, cplx<T> *dst, int L1, hipStream_t s) {
+    constexpr int W = seq_per_wg<T>(LG2, 16);
+    constexpr int NT = fft_threads(LG2, W);
+    auto &tc = TwiddleCache<T>::get();
+    const long N = 1L << sh.lgN;
+    size_t lds = (sizeof(cplx<T>) << LG2) * W;
+    auto k = k_fft_col<T, S, LG2, W, NT, Loader>;
+    set_lds(k, lds);
+    require(L1 % W == 0, "fft column split");
+    hipLaunchKernelGGL(k, dim3(L1 / W, sh.batch), dim3(NT), lds, s, ld, dst, N, L1, tc.fine(LG2),
+                       tc.two_level(sh.lgN));
+    CUNA_LAUNCH_CHECK();
+}
+
+template <typename T, int LG1, class Sink>
+static void run_row_real(const FftShape &sh, const cplx<T> *src, int L2, const Sink &sk,
+                         hipStream_t s) {
+    constexpr int H = seq_per_wg<T>(LG1, 16) / 2 > 0 ? seq_per_wg<T>(LG1, 16) / 2 : 1;
+    constexpr int NT = fft_threads(LG1, 2 * H);
+    auto &tc = TwiddleCache<T>::get();
+    const long N = 1L << sh.lgN;
+    size_t lds = (sizeof(cplx<T>) << LG1) * 2 * H;
+    auto k = k_fft_row_real<T, LG1, H, NT, Sink>;
+    set_lds(k, lds);
+    require(L2 % (2 * H) == 0, "fft row split");
+    hipLaunchKernelGGL(k, dim3(L2 / (2 * H), sh.batch), dim3(NT), lds, s, src, N, L2, sk,
+                       tc.fine(LG1), tc.two_level(sh.lgN + 1));
+    CUNA_LAUNCH_CHECK();
+}
+
+template <typename T, int S, int LG1>
+static void run_row_plain(const FftShape &sh, const cplx<T> *src, int L2, cplx<T> *dst,
+                          hipStream_t s) {
+    constexpr int H = seq_per_wg<T>(LG1, 16);
+    constexpr int NT = fft_threads(LG1, H);
+    auto &tc = TwiddleCache<T>::get();
+    const long N = 1L << sh.lgN;
+    size_t lds = (sizeof(cplx<T>) << LG1) * H;
+    auto k = k_fft_row_plain<T, S, LG1, H, NT>;
+    set_lds(k, lds);
+    require(L2 % H == 0, "fft row split");
+    hipLaunchKernelGGL(k, dim3(L2 / H, sh.batch), dim3(NT), lds, s, src, N, L2, dst, N,
+                       tc.fine(LG1));
+    CUNA_LAUNCH_CHECK();
+}
+
+// compile-time dispatch on a runtime log2 size
+template <int LO, int HI, class F>
+static void dispatch_lg(int lg, F &&f) {
+    if constexpr (LO <= HI) {
+        if (lg == LO) {
+            f(std::integral_constant<int, LO>{});
+            return;
+        }
+        dispatch_lg<LO + 1, HI>(lg, std::forward<F>(f));
+    } else {
+        fail(CUNA_ERROR_UNSUPPORTED, "transform size out of range");
+    }
+}
+
+template <typename T, class Sink>
+static void real_forward_impl(const FftShape &sh, T *grid, const Sink &sk, hipStream_t s) {
+    require(sh.lgN >= 1 && sh.lgN <= max_fft_lg<T>(), "grid size out of range",
+            CUNA_ERROR_UNSUPPORTED);
+    LoadGrid<T> ld;
+    ld.z = reinterpret_cast<const cplx<T> *>(grid);
+    ld.stride = 1L << sh.lgN;
+    if (sh.lgN <= max_small_lg<T>()) {
+        dispatch_lg<1, max_small_lg<T>()>(sh.lgN, [&](auto c) {
+            run_small<T, 1, decltype(c)::value, LoadGrid<T>, Sink, 0>(sh, ld, sk, nullptr, s);
+        });
+        return;
+    }
+    const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
+    cplx<T> *z = reinterpret_cast<cplx<T> *>(grid);
+    dispatch_lg<7, 12>(lg2, [&](auto c) {
+        run_col<T, 1, decltype(c)::value, LoadGrid<T>>(sh, ld, z, 1 << lg1, s);
+    });
+    dispatch_lg<7, 12>(lg1, [&](auto c) {
+        run_row_real<T, decltype(c)::value, Sink>(sh, z, 1 << lg2, sk, s);
+    });
+}
+
+template <typename T>
+void fft_real_forward(const FftShape &sh, T *grid, const SinkAccurate<T> &sk, hipStream_t s) {
+    real_forward_impl<T, SinkAccurate<T>>(sh, grid, sk, s);
+}
+template <typename T>
+void fft_real_forward(const FftShape &sh, T *grid, const SinkCompat<T> &sk, hipStream_t s) {
+    real_forward_impl<T, SinkCompat<T>>(sh, grid, sk, s);
+}
+
+template <typename T>
+void fft_c2r_inverse(const FftShape &sh, const LoadHermitian<T> &ld, cplx<T> *work, T *grid,
+                     hipStream_t s) {
+    require(sh.lgN >= 1 && sh.lgN <= max_fft_lg<T>(), "grid size out of range",
+            CUNA_ERROR_UNSUPPORTED);
+    cplx<T> *z = reinterpret_cast<cplx<T> *>(grid);
+    SinkAccurate<T> dummy{};
+    // the Hermitian loader is register-heavy: the largest single-workgroup
+    // size (1024 threads) would spill, so it takes the two-pass path
+    if (sh.lgN < max_small_lg<T>()) {
+        dispatch_lg<1, max_small_lg<T>() - 1>(sh.lgN, [&](auto c) {
+            run_small<T, -1, decltype(c)::value, LoadHermitian<T>, SinkAccurate<T>, 1>(
+                sh, ld, dummy, z, s);
+        });
+        return;
+    }
+    const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
+    dispatch_lg<7, 12>(lg2, [&](auto c) {
+        run_col<T, -1, decltype(c)::value, LoadHermitian<T>>(sh, ld, work, 1 << lg1, s);
+    });
+    dispatch_lg<7, 12>(lg1, [&](auto c) {
+        run_row_plain<T, -1, decltype(c)::value>(sh, work, 1 << lg2, z, s);
+    });
+}
+
+template <typename T>
+void fft_complex(const FftShape &sh, cplx<T> *data, cplx<T> *work, int sign, hipStream_t s) {
+    require(sh.lgN >= 1 && sh.lgN <= max_fft_lg<T>(), "transform size out of range",
+            CUNA_ERROR_UNSUPPORTED);
+    require(sign == 1 || sign == -1, "sign must be +1 or -1");
+    LoadGrid<T> ld;
+    ld.z = data;
+    ld.stride = 1L << sh.lgN;
+    SinkAccurate<T> dummy{};
+    if (sh.lgN <= max_small_lg<T>()) {
+        dispatch_lg<1, max_small_lg<T>()>(sh.lgN, [&](auto c) {
+            if (sign > 0)
+                run_small<T, 1, decltype(c)::value, LoadGrid<T>, SinkAccurate<T>, 1>(sh, ld, dummy,
+                                                                                     data, s);
+            else
+                run_small<T, -1, decltype(c)::value, LoadGrid<T>, SinkAccurate<T>, 1>(
+                    sh, ld, dummy, data, s);
+        });
+        return;
+    }
+    const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
+    dispatch_lg<7, 12>(lg2, [&](auto c) {
+        if (sign > 0)
+            run_col<T, 1, decltype(c)::value, LoadGrid<T>>(sh, ld, work, 1 << lg1, s);
+        else
+            run_col<T, -1, decltype(c)::value, LoadGrid<T>>(sh, ld, work, 1 << lg1, s);
+    });
+    dispatch_lg<7, 12>(lg1, [&](auto c) {
+        if (sign > 0)
+            run_row_plain<T, 1, decltype(c)::value>(sh, work, 1 << lg2, data, s);
+        else
+            run_row_plain<T, -1, decltype(c)::value>(sh, work, 1 << lg2, data, s);
+    });
+}
+
+// ----------------------------------------------------------------- sort --
+template <typename T>
+void launch_sort(SetupArgs<T> A, const SortWork<T> &W, PointRec<T> *rec, int *perm, int *bucket,
+                 hipStream_t s) {
+    A.chunks = W.chunks;
+    A.mbase = W.mbase;
+    A.nch = W.nch;
+    A.nchunks = W.nchunks;
+    A.keys = W.keys;
+    A.rec_tmp = W.rec_tmp;
+    A.hist = W.hist;
+    // waves per block limited by the per-wave LDS counters
+    size_t per_wave = sizeof(int) * (size_t)A.Kc;
+    require(per_wave <= kMaxLds, "too many grid tiles for one curve", CUNA_ERROR_UNSUPPORTED);
+    int wpb = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per_wave));
+    size_t lds = per_wave * wpb;
+    int nblk = (int)ceil_div(std::max(W.nchunks, 1), wpb);
+    set_lds(k_setup_hist<T>, lds);
+    set_lds(k_scatter<T>, lds);
+    hipLaunchKernelGGL(k_setup_hist<T>, dim3(nblk), dim3(64 * wpb), lds, s, A);
+    CUNA_LAUNCH_CHECK();
+    // exclusive scan of the key-major count matrix
+    const long n = W.hist_len;
+    const long nb = ceil_div(std::max(n, 1L), kScanTile);
+    size_t slds = sizeof(int) * kScanNT;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanNT), slds, s, W.hist, n, W.bsum);
+    CUNA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanNT), slds, s, W.bsum, nb);
+    CUNA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanNT), slds, s, W.hist, n, W.bsum, W.hist);
+    CUNA_LAUNCH_CHECK();
+    ScatterArgs S;
+    S.chunks = W.chunks;
+    S.mbase = W.mbase;
+    S.nch = W.nch;
+    S.nchunks = W.nchunks;
+    S.Kc = A.Kc;
+    S.kbits = ilog2(A.Kc);
+    S.keys = W.keys;
+    S.offsets = W.hist;
+    hipLaunchKernelGGL(k_scatter<T>, dim3(nblk), dim3(64 * wpb), lds, s, S, W.rec_tmp, rec, perm);
+    CUNA_LAUNCH_CHECK();
+    long nbk = (long)W.ncurves * (A.Kc + 1);
+    hipLaunchKernelGGL(k_buckets, dim3(ceil_div(nbk, 256)), dim3(256), 0, s, W.hist, W.mbase, W.nch,
+                       W.coff, W.ncurves, A.Kc, bucket);
+    CUNA_LAUNCH_CHECK();
+}
+
+// --------------------------------------------------------------- spread --
+template <typename T, int NTAPS, int VM>
+static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s) {
+    size_t lds = sizeof(T) * 4 * ((size_t)1 << A.tlog);
+    auto k = k_spread<T, NTAPS, VM>;
+    set_lds(k, lds);
+    hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
+    CUNA_LAUNCH_CHECK();
+}
+
+// accurate filters have 2m taps, m in [2, kMaxTaps/2]; compat filters have
+// 2 R_f - 1 = 11 (float build) or 23 (double build) taps
+template <int LO, int HI, class F>
+static void dispatch_m(int m, F &&f) {
+    if constexpr (LO <= HI) {
+        if (m == LO) {
+            f(std::integral_constant<int, 2 * LO>{});
+            return;
+        }
+        dispatch_m<LO + 1, HI>(m, std::forward<F>(f));
+    } else {
+        fail(CUNA_ERROR_UNSUPPORTED, "unsupported filter width");
+    }
+}
+
+template <typename T>
+void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStream_t s) {
+    if (ntaps == 11 || ntaps == 23) {
+        require(vm != VAL_COUNTS, "counts spread needs an accurate filter");
+        if (ntaps == 11) {
+            if (vm == VAL_PLAIN)
+                spread_one<T, 11, VAL_PLAIN>(A, nslabs, s);
+            else
+                spread_one<T, 11, VAL_COMPAT_BOOT>(A, nslabs, s);
+        } else {
+            if (vm == VAL_PLAIN)
+                spread_one<T, 23, VAL_PLAIN>(A, nslabs, s);
+            else
+                spread_one<T, 23, VAL_COMPAT_BOOT>(A, nslabs, s);
+        }
+        return;
+    }
+    require(ntaps % 2 == 0 && vm != VAL_COMPAT_BOOT, "unsupported filter width",
+            CUNA_ERROR_UNSUPPORTED);
+    dispatch_m<2, kMaxTaps / 2>(ntaps / 2, [&](auto c) {
+        constexpr int NTP = decltype(c)::value;
+        if (vm == VAL_PLAIN)
+            spread_one<T, NTP, VAL_PLAIN>(A, nslabs, s);
+        else
+            spread_one<T, NTP, VAL_COUNTS>(A, nslabs, s);
+    });
+}
+
+template <typename T>
+void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s) {
+    require(ntaps % 2 == 0, "unsupported filter width", CUNA_ERROR_UNSUPPORTED);
+    dispatch_m<2, kMaxTaps / 2>(ntaps / 2, [&](auto c) {
+        constexpr int NTP = decltype(c)::value;
+        size_t lds = sizeof(T) * (((size_t)1 << A.tlog) + NTP);
+        auto k = k_gather<T, NTP>;
+        set_lds(k, lds);
+        hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
+        CUNA_LAUNCH_CHECK();
+    });
+}
+
+// ------------------------------------------------------- instantiation --
+template void fft_real_forward<dTyp>(const FftShape &, dTyp *, const SinkAccurate<dTyp> &,
+                                     hipStream_t);
+template void fft_real_forward<dTyp>(const FftShape &, dTyp *, const SinkCompat<dTyp> &,
+                                     hipStream_t);
+template void fft_c2r_inverse<dTyp>(const FftShape &, const LoadHermitian<dTyp> &, cplx<dTyp> *,
+                                    dTyp *, hipStream_t);
+template void fft_complex<dTyp>(const FftShape &, cplx<dTyp> *, cplx<dTyp> *, int, hipStream_t);
+template void launch_sort<dTyp>(SetupArgs<dTyp>, const SortWork<dTyp> &, PointRec<dTyp> *, int *,
+                                int *, hipStream_t);
+template void launch_spread<dTyp>(const SpreadArgs<dTyp> &, int, int, int, hipStream_t);
+template void launch_gather<dTyp>(const GatherArgs<dTyp> &, int, int, hipStream_t);
+
+}  // namespace cuna

@@ -1,0 +1,70 @@
+// cuna_launch.hpp -- host launchers for the FFT and spread kernels.
+// Defined in cuna_kernels.hip, instantiated for the library's real type.
+#pragma once
+
+#include "cuna_common.hpp"
+#include "cuna_fft.hpp"
+#include "cuna_spread.hpp"
+
+namespace cuna {
+
+// complex length N = 2^lgN of the real grid G = 2N, `batch` transforms
+struct FftShape {
+    int lgN = 0;
+    int batch = 1;
+};
+
+template <typename T>
+constexpr int max_small_lg() {
+    return sizeof(T) == 4 ? 14 : 13;
+}
+template <typename T>
+constexpr int max_fft_lg() {
+    return sizeof(T) == 4 ? 24 : 23;
+}
+
+// Real grid (batch x G reals, read as batch x N complex, modified in place
+// when two passes are used) -> spectrum through the sink.
+template <typename T>
+void fft_real_forward(const FftShape &sh, T *grid, const SinkAccurate<T> &sk, hipStream_t s);
+template <typename T>
+void fft_real_forward(const FftShape &sh, T *grid, const SinkCompat<T> &sk, hipStream_t s);
+
+// Hermitian spectrum (loader) -> real grid (batch x G reals) by an inverse
+// (sign -1) transform; `work` holds batch x N complex.
+template <typename T>
+void fft_c2r_inverse(const FftShape &sh, const LoadHermitian<T> &ld, cplx<T> *work, T *grid,
+                     hipStream_t s);
+
+// Plain complex transform of batch sequences of N = 2^lgN values, in place
+// (used by tests of the FFT stage through the library).
+template <typename T>
+void fft_complex(const FftShape &sh, cplx<T> *data, cplx<T> *work, int sign, hipStream_t s);
+
+// sort-by-tile plan building
+template <typename T>
+struct SortWork {
+    const int4 *chunks;
+    const long *mbase;
+    const int *nch;
+    const int *coff;
+    int nchunks, ncurves;
+    long hist_len;
+    int *keys;
+    PointRec<T> *rec_tmp;
+    int *hist;
+    int *bsum;
+};
+template <typename T>
+void launch_sort(SetupArgs<T> A, const SortWork<T> &W, PointRec<T> *rec, int *perm, int *bucket,
+                 hipStream_t s);
+
+template <typename T>
+void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStream_t s);
+template <typename T>
+void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s);
+
+// bytes of dynamic LDS a launch may use on gfx950
+constexpr size_t kMaxLds = 160 * 1024;
+
+}  // namespace cuna

@@ -1,0 +1,110 @@
+// cuna_plan.hpp -- per-x plan: binned points, filter constants, workspaces.
+//
+// A plan is the MI355X counterpart of the reference's filter_properties +
+// plan pair (inc/cuna_typedefs.h:92-148, src/gaussian_filter.cu:102-164): it
+// is built once per set of sample positions and reused for every y.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "cuna_launch.hpp"
+
+namespace cuna {
+
+// Persistent device allocation (hipMalloc / hipFree).
+struct DevMem {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    void alloc(size_t nbytes) {
+        release();
+        if (nbytes) CUNA_HIP(hipMalloc(&ptr, nbytes));
+        bytes = nbytes;
+    }
+    void ensure(size_t nbytes) {
+        if (nbytes > bytes) alloc(nbytes);
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    ~DevMem() { release(); }
+    template <typename U>
+    U *as() const {
+        return static_cast<U *>(ptr);
+    }
+};
+
+// Points of one or more curves binned by grid tile (stable counting sort).
+template <typename T>
+struct TiledPoints {
+    // host copies of the sort tables, declared first so that they are
+    // destroyed after the device buffers (whose hipFree synchronises)
+    std::vector<int4> chunks_h;
+    std::vector<long> mbase_h;
+    std::vector<int> nch_h;
+    int ncurves = 0;
+    long n = 0;
+    std::vector<int> coff_h;
+    int G = 0, tlog = 0, ntiles = 0, Kc = 0, olo = 0, ntaps = 0;
+    bool periodic = true;
+    DevMem coff, rec, perm, bucket, eps_orig;
+
+    // src: setup arguments with x/t/mode/compat fields filled
+    void build(SetupArgs<T> A, const std::vector<int> &offsets, int G_, int tlog_, int ntaps_,
+               bool periodic_, bool keep_eps, hipStream_t s);
+};
+
+struct ModeSpec {
+    int mode = 0;  // CUNA_MODE_*
+    int sigma = 2, m = 8, tile_log2 = 0, flags = 0;
+};
+
+template <typename T>
+struct Plan {
+    ModeSpec spec;
+    int M = 0;  // accurate: modes per curve; compat: ng
+    int G = 0, lgN = 0, ntaps = 0, olo = 0;
+    T beta = 0, wscale = 1;
+    T e3[kMaxTaps] = {};
+    // compat constants (src/gaussian_filter.cu:54-57)
+    T b_c = 0, normfac_c = 0;
+    int R_f = 0;
+    // kept to identify the x a registered plan belongs to
+    const void *x_key = nullptr;
+    TiledPoints<T> pts, pts2;  // pts2: points 2x (window at 2f)
+    bool has_pts2 = false;
+    DevMem grid, work, counts;
+
+    long n() const { return pts.n; }
+    int ncurves() const { return pts.ncurves; }
+
+    // y -> out (ncurves x M complex)
+    void adjoint(const T *y, cplx<T> *out, hipStream_t s);
+    // compat bootstrap of src/cuna_kernel.cu:390-443 (single curve)
+    void bootstrap_compat(const T *y, cplx<T> *out, int nbs, uint32_t seed, hipStream_t s);
+    // accurate bootstrap (resampled points)
+    void bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx<T> *out_w,
+                   hipStream_t s);
+    void backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s);
+
+    SpreadArgs<T> spread_args(const TiledPoints<T> &P) const;
+    void run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s) const;
+};
+
+template <typename T>
+std::unique_ptr<Plan<T>> make_plan_accurate(const T *x, const double *t, double df,
+                                            const std::vector<int> &offsets, int M,
+                                            const ModeSpec &spec, hipStream_t s);
+template <typename T>
+std::unique_ptr<Plan<T>> make_plan_compat(const T *x, int n, int ng, T *E1, T *E2, T *E3,
+                                          const ModeSpec &spec, hipStream_t s);
+// b and normfac of the compat filter in the build's precision
+template <typename T>
+void compat_constants(int n, int ng, T &b, T &normfac, int &R_f);
+
+}  // namespace cuna

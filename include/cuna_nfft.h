@@ -1,0 +1,100 @@
+/*
+ * cuna_nfft.h -- extensions of the CUNA API with no counterpart in the
+ * reference library (johnh2o2/cunfft_adjoint exports only inc/cuna.h,
+ * inc/cuna_filter.h and init_plan/free_plan).
+ *
+ *   - accurate mode: textbook Greengard-Lee Gaussian NFFT,
+ *       F_k = sum_j y_j exp(2 pi i k x_j),  k = -M/2 .. M/2-1,
+ *     on a grid of G = sigma*M cells with 2m taps (rel. error ~9e-9 at
+ *     sigma=2, m=8 in exact arithmetic; the FP32 library reaches ~2e-7);
+ *   - compat mode: the reference formula (the M outputs are the ng values of
+ *     cunfft_adjoint_raw), usable through the same plan objects;
+ *   - times input: x_j = frac(df * t_j) reduced in FP64 on the GPU, so long
+ *     baselines keep their phase in the FP32 library;
+ *   - batches of curves of uneven length (CSR offsets) in one launch;
+ *   - bootstrap resampling (accurate mode) and its window transform at 2f;
+ *   - backward: gradients of a real loss with respect to y and x;
+ *   - the FFT stage on its own.
+ *
+ * All pointers are device pointers unless stated; every call is ordered on
+ * `stream` and returns a CUNA_* status (also kept in cuna_get_last_error).
+ * A plan executes on one stream at a time: it owns one workspace.
+ */
+#ifndef CUNA_NFFT_H
+#define CUNA_NFFT_H
+#include "cuna_typedefs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cuna_nfft_plan_s *cuna_nfft_plan;
+
+#define CUNA_MODE_ACCURATE 0
+#define CUNA_MODE_COMPAT 1
+
+/* plan flags */
+#define CUNA_PLAN_WINDOW2 0x1 /* also bin the points 2x (window transform at 2f) */
+
+typedef struct {
+    int mode;      /* CUNA_MODE_ACCURATE (default) or CUNA_MODE_COMPAT */
+    int sigma;     /* accurate: oversampling, G = sigma*M, power of two >= 2 (default 2) */
+    int m;         /* accurate: filter half-width, 2m taps, 2..16 (default 8 / 13) */
+    int tile_log2; /* spread tile of 2^tile_log2 cells; 0 = library default */
+    int flags;     /* CUNA_PLAN_* */
+} cuna_nfft_params;
+
+typedef struct {
+    int mode, sigma, m, ntaps;
+    int n, ncurves, M, G;
+    int tile_log2, ntiles;
+    double beta;
+} cuna_nfft_info;
+
+void cuna_nfft_default_params(cuna_nfft_params *p);
+
+/* n points x (accurate: any real, reduced mod 1; compat: [-1/2, 1/2]).
+ * accurate: M modes, M a power of two >= 2; compat: M = ng outputs. */
+int cuna_nfft_plan_create(cuna_nfft_plan *plan, const dTyp *x, int n, int M,
+                          const cuna_nfft_params *params, hipStream_t stream);
+
+/* accurate mode from double times t (device): frequencies k*df, k = -M/2..M/2-1,
+ * x_j = frac(df * t_j) evaluated in FP64 */
+int cuna_nfft_plan_create_times(cuna_nfft_plan *plan, const double *t, int n, double df, int M,
+                                const cuna_nfft_params *params, hipStream_t stream);
+
+/* ncurves curves; curve c holds points offsets[c] .. offsets[c+1]-1 of x.
+ * offsets is a HOST array of ncurves+1 ints with offsets[0] = 0. */
+int cuna_nfft_plan_create_batch(cuna_nfft_plan *plan, const dTyp *x, const int *offsets,
+                                int ncurves, int M, const cuna_nfft_params *params,
+                                hipStream_t stream);
+
+/* out: ncurves x M complex (curve-major).  y: n values or NULL (unit weights). */
+int cuna_nfft_adjoint(cuna_nfft_plan plan, const dTyp *y, Complex *out, hipStream_t stream);
+
+/* accurate mode, per plan: nbs bootstrap replicates.  Replicate s of curve c
+ * draws d = offsets[c] + xorshift(wang(seed + s*n_c + p)) % n_c for
+ * p = 0 .. n_c-1 (the reference's generator, src/gridding.cu:67-101).
+ * out_y[c][s][:]: transform of the values y[d] at points x_d;
+ * out_w[c][s][:] (nullable, needs CUNA_PLAN_WINDOW2): unit weights at 2 x_d. */
+int cuna_nfft_bootstrap(cuna_nfft_plan plan, const dTyp *y, int nbs, unsigned int seed,
+                        Complex *out_y, Complex *out_w, hipStream_t stream);
+
+/* accurate mode: given grad_out = dL/dRe F + i dL/dIm F (ncurves x M complex),
+ * writes dL/dy (n) and dL/dx (n); either output may be NULL.  For a times
+ * plan dL/dt_j = df * dL/dx_j. */
+int cuna_nfft_backward(cuna_nfft_plan plan, const dTyp *y, const Complex *grad_out,
+                       dTyp *grad_y, dTyp *grad_x, hipStream_t stream);
+
+int cuna_nfft_plan_info(cuna_nfft_plan plan, cuna_nfft_info *info);
+int cuna_nfft_plan_destroy(cuna_nfft_plan plan);
+
+/* The FFT stage alone: in-place unnormalised DFT of `batch` sequences of n
+ * (power of two) values, data[r] = sum_g data[g] exp(sign*2 pi i g r/n).
+ * sign = +1 is the reference's CUFFT_INVERSE. */
+int cuna_fft(Complex *data, int n, int batch, int sign, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUNA_NFFT_H */

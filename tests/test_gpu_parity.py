@@ -1,0 +1,316 @@
+"""GPU parity of the HIP library (through its C-ABI) against the CPU oracle.
+
+Tolerances: compat mode vs the oracle's restatement of the reference formula
+rel L2 <= 1e-5 (libcunaf) / 1e-12 (libcunad); accurate mode vs the FP64 direct
+NDFT rel L2 <= 1e-5 (libcunaf, m = 8) / 1e-12 (libcunad, m = 13).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+import cunfft_adjoint_amd as cu  # noqa: E402
+from cunfft_adjoint_amd.autograd import adjoint_nfft  # noqa: E402
+from tests._data import clustered_points, rel_l2, uniform_points  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f": 1e-5, "d": 1e-12}
+NP = {"f": np.float32, "d": np.float64}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.init()
+    # the product must be the HIP library: both .so must load
+    cu.load("f")
+    cu.load("d")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+# ------------------------------------------------------------ FFT stage --
+@pytest.mark.parametrize("prec", ["f", "d"])
+@pytest.mark.parametrize("lg", [1, 2, 3, 4, 5, 8, 10, 12, 13, 14, 15, 16, 18, 20])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_fft_stage_vs_numpy(prec, lg, sign):
+    n = 1 << lg
+    batch = 3 if lg <= 16 else 1
+    rng = np.random.default_rng(lg)
+    z = (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n)))
+    z = z.astype(np.complex64 if prec == "f" else np.complex128)
+    t = dev(z)
+    cu.fft(t, sign=sign)
+    got = host(t)
+    ref = np.fft.ifft(z.astype(np.complex128), axis=-1) * n if sign > 0 else np.fft.fft(
+        z.astype(np.complex128), axis=-1)
+    assert rel_l2(got, ref) < (2e-6 if prec == "f" else 1e-13)
+
+
+# --------------------------------------------------------- compat mode --
+@pytest.mark.parametrize("prec", ["f", "d"])
+@pytest.mark.parametrize("n,ng", [(10000, 1 << 15), (1000, 1 << 10), (7, 16), (1, 4), (300, 256)])
+def test_compat_raw_vs_oracle(prec, n, ng):
+    x, y = uniform_points(n, 11 + n, NP[prec])
+    xd, yd = dev(x), dev(y)
+    fp = cu.generate_filter_properties(xd, n, ng)
+    f_hat = torch.zeros(ng, dtype=torch.complex64 if prec == "f" else torch.complex128,
+                        device="cuda")
+    cu.cunfft_adjoint_raw(xd, yd, f_hat, n, ng, fp)
+    got = host(f_hat)
+    ref = oracle.compat_adjoint(x, y, ng, prec == "d")
+    assert rel_l2(got, ref) < TOL[prec]
+    # filter constants and E1/E2/E3 as src/gaussian_filter.cu:54-57,204-228
+    b, nf, rf = oracle.compat_params(n, ng, prec == "d")
+    fl = fp.fields
+    assert fl.filter_radius == rf
+    assert fl.b == pytest.approx(b, rel=1e-7 if prec == "f" else 1e-15)
+    assert fl.normfac == pytest.approx(nf, rel=1e-7 if prec == "f" else 1e-15)
+    R = NP[prec]
+    e1 = np.empty(n, R)
+    e2 = np.empty(n, R)
+    e3 = np.empty(rf, R)
+    torch.cuda.synchronize()
+    _copy_d2h(e1, fl.E1)
+    _copy_d2h(e2, fl.E2)
+    _copy_d2h(e3, fl.E3)
+    eps = np.array([oracle.compat_point(xi, n, ng, prec == "d")[1] for xi in x[:200]])
+    tol = 2e-6 if prec == "f" else 1e-13
+    np.testing.assert_allclose(e1[:200], np.exp(-eps * eps / b) / np.sqrt(np.pi * b), rtol=tol)
+    np.testing.assert_allclose(e2[:200], np.exp(2 * eps / b), rtol=tol)
+    np.testing.assert_allclose(e3, np.exp(-np.arange(rf) ** 2 / b), rtol=tol)
+    fp.free()
+
+
+def _copy_d2h(dst, ptr):
+    import ctypes
+    lib = ctypes.CDLL("libamdhip64.so")
+    lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert lib.hipMemcpy(dst.ctypes.data, ptr, dst.nbytes, 2) == 0
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_compat_async_and_bootstrap_vs_oracle(prec):
+    n, ng, nbs, seed = 3000, 1 << 13, 5, 12345
+    x, y = uniform_points(n, 3, NP[prec])
+    xd, yd = dev(x), dev(y)
+    fp = cu.generate_filter_properties(xd, n, ng)
+    s = torch.cuda.Stream()
+    cdt = torch.complex64 if prec == "f" else torch.complex128
+    f_hat = torch.zeros(ng, dtype=cdt, device="cuda")
+    cu.cunfft_adjoint_raw_async(xd, yd, f_hat, n, ng, fp, stream=s)
+    s.synchronize()
+    assert rel_l2(host(f_hat), oracle.compat_adjoint(x, y, ng, prec == "d")) < TOL[prec]
+    fb = torch.zeros(nbs * ng, dtype=cdt, device="cuda")
+    cu.cunfft_adjoint_raw_async_bootstrap(xd, yd, fb, n, ng, nbs, fp, stream=s, seed=seed)
+    s.synchronize()
+    ref = oracle.compat_adjoint_bootstrap(x, y, ng, nbs, seed, prec == "d")
+    assert rel_l2(host(fb).reshape(nbs, ng), ref) < TOL[prec]
+    # f_data == NULL: unit weights, no draw (src/gridding.cu:98-99)
+    fb.zero_()
+    cu.cunfft_adjoint_raw_async_bootstrap(xd, None, fb, n, ng, nbs, fp, stream=s, seed=seed)
+    s.synchronize()
+    ref = oracle.compat_adjoint_bootstrap(x, None, ng, nbs, seed, prec == "d")
+    assert rel_l2(host(fb).reshape(nbs, ng), ref) < TOL[prec]
+    fp.free()
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_compat_host_api_and_plan_path(prec):
+    n, ng = 2000, 1 << 12
+    x, y = uniform_points(n, 5, NP[prec])
+    got = cu.cunfft_adjoint(x, y, ng)
+    ref = oracle.compat_adjoint(x, y, ng, prec == "d")
+    assert rel_l2(got, ref) < TOL[prec]
+    plan = cu.CunaPlan(x, y, ng, cu._lib.CALCULATE_WINDOW_FUNCTION)
+    plan.execute()
+    assert rel_l2(plan.f_hat(), ref) < TOL[prec]
+    assert rel_l2(plan.f_hat_win(), oracle.compat_adjoint(x, None, ng, prec == "d")) < TOL[prec]
+    plan.free()
+
+
+def test_compat_unregistered_fprops_rebuilds():
+    n, ng = 500, 1 << 10
+    x, y = uniform_points(n, 9)
+    xd, yd = dev(x), dev(y)
+    f_hat = torch.zeros(ng, dtype=torch.complex64, device="cuda")
+    cu.cunfft_adjoint_raw(xd, yd, f_hat, n, ng, 0)
+    assert rel_l2(host(f_hat), oracle.compat_adjoint(x, y, ng, False)) < 1e-5
+
+
+# ------------------------------------------------------- accurate mode --
+@pytest.mark.parametrize("prec,m", [("f", 8), ("d", 13)])
+def test_accurate_c1_vs_ndft(prec, m):
+    n, M = 10000, 1 << 14
+    x, y = uniform_points(n, 21, NP[prec])
+    plan = cu.NFFTPlan(dev(x), M, m=m)
+    got = host(plan.adjoint(dev(y)))[0]
+    ref = oracle.ndft(x.astype(np.float64), y.astype(np.float64), M)
+    err = rel_l2(got, ref)
+    assert err < TOL[prec], err
+    # same algorithm in FP64 on the CPU
+    gl = oracle.gl_adjoint(x, y, M, 2, m)
+    assert rel_l2(got, gl) < (1e-5 if prec == "f" else 1e-12)
+
+
+def test_accurate_c2_bands_vs_ndft():
+    n, M = 1_000_000, 1 << 20
+    x, y = clustered_points(n, 2024)
+    plan = cu.NFFTPlan(dev(x), M)
+    got = host(plan.adjoint(dev(y)))[0]
+    rng = np.random.default_rng(7)
+    k0 = np.sort(rng.integers(-M // 2, M // 2 - 64, 6))
+    k0 = np.concatenate([k0, [-M // 2, -32, M // 2 - 64]])
+    ref = oracle.ndft_bands(x.astype(np.float64), y.astype(np.float64), k0, 64)
+    sel = np.stack([got[k + M // 2: k + M // 2 + 64] for k in k0])
+    assert rel_l2(sel, ref) < 1e-5
+    # property at full size: F_0 = sum y, Hermitian symmetry for real y
+    assert abs(got[M // 2] - y.astype(np.float64).sum()) < 1e-5 * np.abs(got).max()
+    assert rel_l2(got[1:M // 2][::-1], np.conj(got[M // 2 + 1:])) < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_accurate_edge_cases(prec):
+    R = NP[prec]
+    M = 64
+    # one point at 0 with y = 1: F == 1
+    plan = cu.NFFTPlan(dev(np.zeros(1, R)), M)
+    got = host(plan.adjoint(dev(np.ones(1, R))))[0]
+    np.testing.assert_allclose(got, np.ones(M), atol=TOL[prec] * 10)
+    # all points in one cell (one hot tile), x = -1/2 and x just below 1/2
+    x = np.array([0.123] * 500 + [-0.5, np.nextafter(R(0.5), R(0))] * 10, dtype=R)
+    y = np.random.default_rng(0).standard_normal(len(x)).astype(R)
+    for MM in (64, 1 << 12):
+        plan = cu.NFFTPlan(dev(x), MM)
+        got = host(plan.adjoint(dev(y)))[0]
+        ref = oracle.ndft(x.astype(np.float64), y.astype(np.float64), MM)
+        assert rel_l2(got, ref) < TOL[prec]
+    # unit weights (y = NULL)
+    got = host(plan.adjoint(None))[0]
+    assert rel_l2(got, oracle.ndft(x.astype(np.float64), None, 1 << 12)) < TOL[prec]
+
+
+def test_accurate_small_grids_and_tiles():
+    for M, tl in ((2, 0), (4, 0), (16, 0), (256, 6), (1 << 12, 7), (1 << 15, 9)):
+        x, y = uniform_points(777, M)
+        plan = cu.NFFTPlan(dev(x), M, tile_log2=tl)
+        got = host(plan.adjoint(dev(y)))[0]
+        ref = oracle.ndft(x.astype(np.float64), y.astype(np.float64), M)
+        assert rel_l2(got, ref) < 1e-5, (M, tl)
+
+
+def test_determinism_bitwise():
+    x, y = clustered_points(200_000, 5)
+    plan = cu.NFFTPlan(dev(x), 1 << 18)
+    a = host(plan.adjoint(dev(y)))
+    b = host(plan.adjoint(dev(y)))
+    plan2 = cu.NFFTPlan(dev(x), 1 << 18)
+    c = host(plan2.adjoint(dev(y)))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
+def test_batch_equals_single_curves():
+    rng = np.random.default_rng(3)
+    lens = [0, 1, 100, 5000, 20000, 333]
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    x = rng.uniform(-0.5, 0.5, offs[-1]).astype(np.float32)
+    y = rng.standard_normal(offs[-1]).astype(np.float32)
+    M = 1 << 13
+    bp = cu.NFFTPlan(dev(x), M, offsets=offs)
+    got = host(bp.adjoint(dev(y)))
+    for c in range(len(lens)):
+        xs, ys = x[offs[c]:offs[c + 1]], y[offs[c]:offs[c + 1]]
+        if len(xs) == 0:
+            assert np.all(got[c] == 0)
+            continue
+        single = host(cu.NFFTPlan(dev(xs), M).adjoint(dev(ys)))[0]
+        assert rel_l2(got[c], single) < 1e-6
+        ref = oracle.ndft(xs.astype(np.float64), ys.astype(np.float64), M)
+        assert rel_l2(got[c], ref) < 1e-5
+
+
+def test_times_plan_fp64_phase_reduction():
+    # long baseline: t in [0, 36500) days, df = 1/(5*baseline)
+    rng = np.random.default_rng(4)
+    n = 20000
+    t = np.sort(rng.uniform(0, 36500, n))
+    y = rng.standard_normal(n).astype(np.float32)
+    df = 1.0 / (5 * 36500.0)
+    M = 1 << 16
+    plan = cu.NFFTPlan(t=dev(t), df=df, M=M, dtype=torch.float32)
+    got = host(plan.adjoint(dev(y)))[0]
+    xs = (df * t) % 1.0
+    ref = oracle.ndft(xs, y.astype(np.float64), M)
+    assert rel_l2(got, ref) < 1e-5
+    # an FP32 x loses the phase at this baseline (why the times path exists)
+    x32 = ((df * t + 0.5) % 1.0 - 0.5).astype(np.float32)
+    bad = host(cu.NFFTPlan(dev(x32), M).adjoint(dev(y)))[0]
+    assert rel_l2(bad, ref) > rel_l2(got, ref)
+
+
+# ------------------------------------------------------------ bootstrap --
+def test_accurate_bootstrap_vs_oracle():
+    n, M, nbs, seed = 2000, 1 << 11, 4, 777
+    x, y = uniform_points(n, 8)
+    plan = cu.NFFTPlan(dev(x), M, window2=True)
+    oy, ow = plan.bootstrap(dev(y), nbs, seed, window=True)
+    oy, ow = host(oy), host(ow)
+    d = oracle.bootstrap_indices(seed, n, nbs)
+    xd = x.astype(np.float64)
+    for s in range(nbs):
+        ref = oracle.ndft(xd[d[s]], y.astype(np.float64)[d[s]], M)
+        assert rel_l2(oy[s], ref) < 1e-5
+        refw = oracle.ndft(2 * xd[d[s]], None, M)
+        assert rel_l2(ow[s], refw) < 1e-5
+
+
+# ------------------------------------------------------------- backward --
+@pytest.mark.parametrize("prec,m", [("f", 8), ("d", 13)])
+def test_backward_vs_oracle(prec, m):
+    n, M = 3000, 1 << 10
+    x, y = uniform_points(n, 31, NP[prec])
+    rng = np.random.default_rng(32)
+    g = (rng.standard_normal(M) + 1j * rng.standard_normal(M))
+    plan = cu.NFFTPlan(dev(x), M, m=m)
+    gd = dev(g.astype(np.complex64 if prec == "f" else np.complex128)).reshape(1, M)
+    dy, dx = plan.backward(dev(y), gd)
+    dy, dx = host(dy), host(dx)
+    rdy, rdx = oracle.gl_grad(x, y, M, g, 2, m)
+    tol = 1e-5 if prec == "f" else 1e-11
+    assert rel_l2(dy, rdy) < tol
+    assert rel_l2(dx, rdx) < tol
+    ady, adx = oracle.ndft_grad(x, y, M, g)
+    tol = 1e-5 if prec == "f" else 1e-11
+    assert rel_l2(dy, ady) < tol
+    assert rel_l2(dx, adx) < tol
+
+
+def test_autograd_batched_loss():
+    B, n, M = 4, 2000, 1 << 12
+    rng = np.random.default_rng(40)
+    x = torch.tensor(rng.uniform(-0.5, 0.5, (B, n)), dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    y = torch.tensor(rng.standard_normal((B, n)), dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    w = torch.tensor(rng.uniform(0, 1, M), dtype=torch.float64, device="cuda")
+    F = adjoint_nfft(x, y, M, m=13)
+    L = (w * (F.real ** 2 + F.imag ** 2)).sum()
+    L.backward()
+    xh, yh = x.detach().cpu().numpy(), y.detach().cpu().numpy()
+    for b in range(B):
+        Fb = oracle.ndft(xh[b], yh[b], M)
+        g = 2 * w.cpu().numpy() * Fb
+        ady, adx = oracle.ndft_grad(xh[b], yh[b], M, g)
+        assert rel_l2(y.grad[b].cpu().numpy(), ady) < 1e-10
+        assert rel_l2(x.grad[b].cpu().numpy(), adx) < 1e-10
