@@ -106,6 +106,7 @@ def _protos(prec: str):
         "cuna_nfft_bootstrap": (c_int, [P, P, c_int, c_uint, P, P, P]),
         "cuna_nfft_backward": (c_int, [P, P, P, P, P, P]),
         "cuna_nfft_plan_info": (c_int, [P, P]),
+        "cuna_nfft_set_profile_events": (c_int, [P, P, c_int]),
         "cuna_nfft_plan_destroy": (c_int, [P]),
         "cuna_fft": (c_int, [P, c_int, c_int, c_int, P]),
     }

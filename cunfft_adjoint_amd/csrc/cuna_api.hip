@@ -219,7 +219,7 @@ SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
 }
 
 template <typename T>
-void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s) const {
+void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s, hipEvent_t mid) const {
     FftShape sh;
     sh.lgN = lgN;
     sh.batch = batch;
@@ -232,7 +232,7 @@ void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s) const
         const double pi = 3.14159265358979323846;
         sk.coef = (T)(pi * pi * (double)beta);
         sk.pref = (T)(1.0 / std::sqrt(pi * (double)beta));
-        fft_real_forward<T>(sh, grid_ptr, sk, s);
+        fft_real_forward<T>(sh, grid_ptr, sk, s, mid);
     } else {
         SinkCompat<T> sk;
         sk.out = out;
@@ -241,7 +241,7 @@ void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s) const
         sk.invG = T(1) / (T)G;
         sk.qb = b_c * T(0.25);
         sk.normfac = normfac_c;
-        fft_real_forward<T>(sh, grid_ptr, sk, s);
+        fft_real_forward<T>(sh, grid_ptr, sk, s, mid);
     }
 }
 
@@ -254,8 +254,11 @@ void Plan<T>::adjoint(const T *y, cplx<T> *out, hipStream_t s) {
     A.y = y;
     A.grid = grid.as<T>();
     A.grid_stride = G;
+    if (prof[0]) CUNA_HIP(hipEventRecord(prof[0], s));
     launch_spread<T>(A, ntaps, VAL_PLAIN, nc, s);
-    run_fft(grid.as<T>(), out, nc, s);
+    if (prof[1]) CUNA_HIP(hipEventRecord(prof[1], s));
+    run_fft(grid.as<T>(), out, nc, s, prof[2]);
+    if (prof[3]) CUNA_HIP(hipEventRecord(prof[3], s));
 }
 
 template <typename T>
@@ -358,9 +361,11 @@ void Plan<T>::backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s
     launch_gather<T>(A, ntaps, nc, s);
 }
 
+// 512-cell tiles: many small workgroups keep the whole chip busy and bound
+// the work of the densest tile
 template <typename T>
 static int default_tlog() {
-    return sizeof(T) == 4 ? 11 : 10;
+    return 9;
 }
 
 template <typename T>
@@ -532,6 +537,13 @@ static std::shared_ptr<Plan<R>> compat_plan_for(const R *x, int n, int ng,
     return std::shared_ptr<Plan<R>>(std::move(fresh));
 }
 
+// the reference accepts any ng for cuFFT; this build's FFT takes powers of two
+static void check_compat_sizes(int n, int ng) {
+    require(n > 0 && ng > 0, "n and ng must be positive");
+    require(ng >= 4 && is_pow2(ng), "ng must be a power of two >= 4", CUNA_ERROR_UNSUPPORTED);
+    require((long)n < 2L * ng, "n must be smaller than 2*ng (the filter width b is not positive)");
+}
+
 static void generate_impl(const R *x, int n, int ng, filter_properties *h, filter_properties *d,
                           hipStream_t s, bool sync) {
     require(h != nullptr && d != nullptr, "filter_properties pointers are NULL");
@@ -569,7 +581,8 @@ CUNA_API void generate_filter_properties(const dTyp *x, const int n, const int n
                                          filter_properties **fprops_device) {
     guarded([&] {
         require(fprops_host && fprops_device, "output pointers are NULL");
-        require(n > 0 && ng > 0, "n and ng must be positive");
+        check_compat_sizes(n, ng);
+        require(x != nullptr, "x is NULL");
         const int rf = cuna_filter_radius();
         filter_properties *h = (filter_properties *)malloc(sizeof(filter_properties));
         require(h != nullptr, "host allocation failed", CUNA_ERROR_OUT_OF_MEMORY);
@@ -668,7 +681,7 @@ CUNA_API void init_plan(plan *p, const dTyp *x, const dTyp *f, int Ndata, int Ng
     guarded([&] {
         require(p != nullptr, "plan is NULL");
         require(x != nullptr, "x is NULL");
-        require(Ndata > 0 && Ngrid > 0, "Ndata and Ngrid must be positive");
+        check_compat_sizes(Ndata, Ngrid);
         memset(p, 0, sizeof *p);
         p->Ndata = Ndata;
         p->Ngrid = Ngrid;
@@ -890,6 +903,14 @@ CUNA_API int cuna_nfft_plan_info(cuna_nfft_plan plan, cuna_nfft_info *info) {
         info->tile_log2 = p.pts.tlog;
         info->ntiles = p.pts.ntiles;
         info->beta = (double)p.beta;
+    });
+}
+
+CUNA_API int cuna_nfft_set_profile_events(cuna_nfft_plan plan, hipEvent_t *events, int nevents) {
+    return guarded([&] {
+        require(plan && plan->p, "plan is NULL");
+        require(nevents >= 0 && nevents <= 4 && (nevents == 0 || events), "bad event list");
+        for (int i = 0; i < 4; ++i) plan->p->prof[i] = i < nevents ? events[i] : nullptr;
     });
 }
 

@@ -1,5 +1,8 @@
 // cuna_kernels.hip -- instantiation and launch of the FFT and spread kernels
 // for the library's real type (float in libcunaf.so, double in libcunad.so).
+#include <algorithm>
+#include <cstdlib>
+
 #include "cuna_launch.hpp"
 
 namespace cuna {
@@ -40,9 +43,8 @@ static void run_small(const FftShape &sh, const Loader &ld, const Sink &sk, cplx
     CUNA_LAUNCH_CHECK();
 }
 
-template <typename T, int S, int LG2, class Loader>
+template <typename T, int S, int LG2, int W, class Loader>
 static void run_col(const FftShape &sh, const Loader &ld, cplx<T> *dst, int L1, hipStream_t s) {
-    constexpr int W = seq_per_wg<T>(LG2, 16);
     constexpr int NT = fft_threads(LG2, W);
     auto &tc = TwiddleCache<T>::get();
     const long N = 1L << sh.lgN;
@@ -55,10 +57,9 @@ static void run_col(const FftShape &sh, const Loader &ld, cplx<T> *dst, int L1, 
     CUNA_LAUNCH_CHECK();
 }
 
-template <typename T, int LG1, class Sink>
+template <typename T, int LG1, int H, class Sink>
 static void run_row_real(const FftShape &sh, const cplx<T> *src, int L2, const Sink &sk,
                          hipStream_t s) {
-    constexpr int H = seq_per_wg<T>(LG1, 16) / 2 > 0 ? seq_per_wg<T>(LG1, 16) / 2 : 1;
     constexpr int NT = fft_threads(LG1, 2 * H);
     auto &tc = TwiddleCache<T>::get();
     const long N = 1L << sh.lgN;
@@ -71,10 +72,9 @@ static void run_row_real(const FftShape &sh, const cplx<T> *src, int L2, const S
     CUNA_LAUNCH_CHECK();
 }
 
-template <typename T, int S, int LG1>
+template <typename T, int S, int LG1, int H>
 static void run_row_plain(const FftShape &sh, const cplx<T> *src, int L2, cplx<T> *dst,
                           hipStream_t s) {
-    constexpr int H = seq_per_wg<T>(LG1, 16);
     constexpr int NT = fft_threads(LG1, H);
     auto &tc = TwiddleCache<T>::get();
     const long N = 1L << sh.lgN;
@@ -85,6 +85,60 @@ static void run_row_plain(const FftShape &sh, const cplx<T> *src, int L2, cplx<T
     hipLaunchKernelGGL(k, dim3(L2 / H, sh.batch), dim3(NT), lds, s, src, N, L2, dst, N,
                        tc.fine(LG1));
     CUNA_LAUNCH_CHECK();
+}
+
+// Sequences per workgroup of a pass: the largest count whose LDS tile fits in
+// 64 KB that still gives >= 512 workgroups (two or more per CU); an
+// environment override (CUNA_FFT_W, CUNA_FFT_H) exists for tuning runs.
+template <typename T, int LG, int W>
+constexpr bool lds_fits() {
+    return (long)W * (1L << LG) * (long)sizeof(cplx<T>) <= 65536;
+}
+
+static int env_int(const char *name) {
+    const char *v = getenv(name);
+    return v ? atoi(v) : 0;
+}
+
+template <typename T>
+static int pick_seq(int lg, long nseq, int batch, int per_wg_mult, const char *env) {
+    int forced = env_int(env);
+    int best = 1;
+    for (int w = 8; w >= 1; w >>= 1) {
+        long lds = (long)w * per_wg_mult * (1L << lg) * (long)sizeof(cplx<T>);
+        if (lds > 65536 || nseq % (w * per_wg_mult) != 0) continue;
+        if (forced) {
+            if (w == forced) return w;
+            continue;
+        }
+        if ((nseq / (w * per_wg_mult)) * (long)batch >= 512) return w;
+        best = std::min(best, w);
+    }
+    return best;
+}
+
+template <typename T, int LG, class F>
+static void dispatch_w(int w, F &&f) {
+    if constexpr (lds_fits<T, LG, 8>())
+        if (w == 8) return f(std::integral_constant<int, 8>{});
+    if constexpr (lds_fits<T, LG, 4>())
+        if (w == 4) return f(std::integral_constant<int, 4>{});
+    if constexpr (lds_fits<T, LG, 2>())
+        if (w == 2) return f(std::integral_constant<int, 2>{});
+    if (w == 1) return f(std::integral_constant<int, 1>{});
+    fail(CUNA_ERROR_UNSUPPORTED, "fft workgroup shape");
+}
+
+// dispatch for the row pass with mirror pairs (2H sequences per workgroup)
+template <typename T, int LG, class F>
+static void dispatch_h(int h, F &&f) {
+    if constexpr (lds_fits<T, LG, 8>())
+        if (h == 4) return f(std::integral_constant<int, 4>{});
+    if constexpr (lds_fits<T, LG, 4>())
+        if (h == 2) return f(std::integral_constant<int, 2>{});
+    if constexpr (lds_fits<T, LG, 2>())
+        if (h == 1) return f(std::integral_constant<int, 1>{});
+    fail(CUNA_ERROR_UNSUPPORTED, "fft workgroup shape");
 }
 
 // compile-time dispatch on a runtime log2 size
@@ -102,7 +156,8 @@ static void dispatch_lg(int lg, F &&f) {
 }
 
 template <typename T, class Sink>
-static void real_forward_impl(const FftShape &sh, T *grid, const Sink &sk, hipStream_t s) {
+static void real_forward_impl(const FftShape &sh, T *grid, const Sink &sk, hipStream_t s,
+                              hipEvent_t mid) {
     require(sh.lgN >= 1 && sh.lgN <= max_fft_lg<T>(), "grid size out of range",
             CUNA_ERROR_UNSUPPORTED);
     LoadGrid<T> ld;
@@ -116,21 +171,32 @@ static void real_forward_impl(const FftShape &sh, T *grid, const Sink &sk, hipSt
     }
     const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
     cplx<T> *z = reinterpret_cast<cplx<T> *>(grid);
+    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W");
+    const int H = std::min(4, pick_seq<T>(lg1, 1L << lg2, sh.batch, 2, "CUNA_FFT_H"));
     dispatch_lg<7, 12>(lg2, [&](auto c) {
-        run_col<T, 1, decltype(c)::value, LoadGrid<T>>(sh, ld, z, 1 << lg1, s);
+        constexpr int LG2 = decltype(c)::value;
+        dispatch_w<T, LG2>(W, [&](auto w) {
+            run_col<T, 1, LG2, decltype(w)::value, LoadGrid<T>>(sh, ld, z, 1 << lg1, s);
+        });
     });
+    if (mid) CUNA_HIP(hipEventRecord(mid, s));
     dispatch_lg<7, 12>(lg1, [&](auto c) {
-        run_row_real<T, decltype(c)::value, Sink>(sh, z, 1 << lg2, sk, s);
+        constexpr int LG1 = decltype(c)::value;
+        dispatch_h<T, LG1>(H, [&](auto h) {
+            run_row_real<T, LG1, decltype(h)::value, Sink>(sh, z, 1 << lg2, sk, s);
+        });
     });
 }
 
 template <typename T>
-void fft_real_forward(const FftShape &sh, T *grid, const SinkAccurate<T> &sk, hipStream_t s) {
-    real_forward_impl<T, SinkAccurate<T>>(sh, grid, sk, s);
+void fft_real_forward(const FftShape &sh, T *grid, const SinkAccurate<T> &sk, hipStream_t s,
+                      hipEvent_t mid) {
+    real_forward_impl<T, SinkAccurate<T>>(sh, grid, sk, s, mid);
 }
 template <typename T>
-void fft_real_forward(const FftShape &sh, T *grid, const SinkCompat<T> &sk, hipStream_t s) {
-    real_forward_impl<T, SinkCompat<T>>(sh, grid, sk, s);
+void fft_real_forward(const FftShape &sh, T *grid, const SinkCompat<T> &sk, hipStream_t s,
+                      hipEvent_t mid) {
+    real_forward_impl<T, SinkCompat<T>>(sh, grid, sk, s, mid);
 }
 
 template <typename T>
@@ -150,11 +216,19 @@ void fft_c2r_inverse(const FftShape &sh, const LoadHermitian<T> &ld, cplx<T> *wo
         return;
     }
     const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
+    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W");
+    const int H = pick_seq<T>(lg1, 1L << lg2, sh.batch, 1, "CUNA_FFT_H");
     dispatch_lg<7, 12>(lg2, [&](auto c) {
-        run_col<T, -1, decltype(c)::value, LoadHermitian<T>>(sh, ld, work, 1 << lg1, s);
+        constexpr int LG2 = decltype(c)::value;
+        dispatch_w<T, LG2>(W, [&](auto w) {
+            run_col<T, -1, LG2, decltype(w)::value, LoadHermitian<T>>(sh, ld, work, 1 << lg1, s);
+        });
     });
     dispatch_lg<7, 12>(lg1, [&](auto c) {
-        run_row_plain<T, -1, decltype(c)::value>(sh, work, 1 << lg2, z, s);
+        constexpr int LG1 = decltype(c)::value;
+        dispatch_w<T, LG1>(H, [&](auto h) {
+            run_row_plain<T, -1, LG1, decltype(h)::value>(sh, work, 1 << lg2, z, s);
+        });
     });
 }
 
@@ -179,17 +253,27 @@ void fft_complex(const FftShape &sh, cplx<T> *data, cplx<T> *work, int sign, hip
         return;
     }
     const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
+    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W");
+    const int H = pick_seq<T>(lg1, 1L << lg2, sh.batch, 1, "CUNA_FFT_H");
     dispatch_lg<7, 12>(lg2, [&](auto c) {
-        if (sign > 0)
-            run_col<T, 1, decltype(c)::value, LoadGrid<T>>(sh, ld, work, 1 << lg1, s);
-        else
-            run_col<T, -1, decltype(c)::value, LoadGrid<T>>(sh, ld, work, 1 << lg1, s);
+        constexpr int LG2 = decltype(c)::value;
+        dispatch_w<T, LG2>(W, [&](auto w) {
+            constexpr int WW = decltype(w)::value;
+            if (sign > 0)
+                run_col<T, 1, LG2, WW, LoadGrid<T>>(sh, ld, work, 1 << lg1, s);
+            else
+                run_col<T, -1, LG2, WW, LoadGrid<T>>(sh, ld, work, 1 << lg1, s);
+        });
     });
     dispatch_lg<7, 12>(lg1, [&](auto c) {
-        if (sign > 0)
-            run_row_plain<T, 1, decltype(c)::value>(sh, work, 1 << lg2, data, s);
-        else
-            run_row_plain<T, -1, decltype(c)::value>(sh, work, 1 << lg2, data, s);
+        constexpr int LG1 = decltype(c)::value;
+        dispatch_w<T, LG1>(H, [&](auto h) {
+            constexpr int HH = decltype(h)::value;
+            if (sign > 0)
+                run_row_plain<T, 1, LG1, HH>(sh, work, 1 << lg2, data, s);
+            else
+                run_row_plain<T, -1, LG1, HH>(sh, work, 1 << lg2, data, s);
+        });
     });
 }
 
@@ -309,9 +393,9 @@ void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s)
 
 // ------------------------------------------------------- instantiation --
 template void fft_real_forward<dTyp>(const FftShape &, dTyp *, const SinkAccurate<dTyp> &,
-                                     hipStream_t);
+                                     hipStream_t, hipEvent_t);
 template void fft_real_forward<dTyp>(const FftShape &, dTyp *, const SinkCompat<dTyp> &,
-                                     hipStream_t);
+                                     hipStream_t, hipEvent_t);
 template void fft_c2r_inverse<dTyp>(const FftShape &, const LoadHermitian<dTyp> &, cplx<dTyp> *,
                                     dTyp *, hipStream_t);
 template void fft_complex<dTyp>(const FftShape &, cplx<dTyp> *, cplx<dTyp> *, int, hipStream_t);

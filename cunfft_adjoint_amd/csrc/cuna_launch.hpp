@@ -25,10 +25,13 @@ constexpr int max_fft_lg() {
 
 // Real grid (batch x G reals, read as batch x N complex, modified in place
 // when two passes are used) -> spectrum through the sink.
+// `mid` (optional) is recorded on s between the two passes of a two-pass FFT.
 template <typename T>
-void fft_real_forward(const FftShape &sh, T *grid, const SinkAccurate<T> &sk, hipStream_t s);
+void fft_real_forward(const FftShape &sh, T *grid, const SinkAccurate<T> &sk, hipStream_t s,
+                      hipEvent_t mid = nullptr);
 template <typename T>
-void fft_real_forward(const FftShape &sh, T *grid, const SinkCompat<T> &sk, hipStream_t s);
+void fft_real_forward(const FftShape &sh, T *grid, const SinkCompat<T> &sk, hipStream_t s,
+                      hipEvent_t mid = nullptr);
 
 // Hermitian spectrum (loader) -> real grid (batch x G reals) by an inverse
 // (sign -1) transform; `work` holds batch x N complex.

@@ -79,6 +79,9 @@ struct Plan {
     TiledPoints<T> pts, pts2;  // pts2: points 2x (window at 2f)
     bool has_pts2 = false;
     DevMem grid, work, counts;
+    // optional profiling events, recorded on the execution stream before the
+    // first kernel of adjoint() and after each of its kernels
+    hipEvent_t prof[4] = {nullptr, nullptr, nullptr, nullptr};
 
     long n() const { return pts.n; }
     int ncurves() const { return pts.ncurves; }
@@ -93,7 +96,8 @@ struct Plan {
     void backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s);
 
     SpreadArgs<T> spread_args(const TiledPoints<T> &P) const;
-    void run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s) const;
+    void run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s,
+                 hipEvent_t mid = nullptr) const;
 };
 
 template <typename T>

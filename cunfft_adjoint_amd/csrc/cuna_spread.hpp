@@ -353,14 +353,10 @@ __device__ __forceinline__ void tap_weights(T v, T e, T bi, const T *e3, T *w) {
     }
 }
 
-template <typename T, int NTAPS, int VM>
-__device__ __forceinline__ void spread_point(const SpreadArgs<T> &A, T *acc, int TT, long s,
-                                             int off, int c, int rep) {
-    constexpr int Q0 = (NTAPS - 1) / 2;
-    const PointRec<T> pr = A.rec[s];
-    const int p = A.perm[s];
-    T e = pr.e;
-    T v;
+// value and weight parameter of sorted point s (original index p)
+template <typename T, int VM>
+__device__ __forceinline__ void point_value(const SpreadArgs<T> &A, int p, int c, int rep, T &v,
+                                            T &e) {
     if constexpr (VM == VAL_PLAIN) {
         v = A.y ? A.y[p + rep * A.y_slab_stride] : T(1);
     } else if constexpr (VM == VAL_COMPAT_BOOT) {
@@ -376,28 +372,60 @@ __device__ __forceinline__ void spread_point(const SpreadArgs<T> &A, T *acc, int
         }
     } else {
         v = (T)A.counts[p + rep * A.counts_stride] * (A.y ? A.y[p] : T(1));
-        if (v == T(0)) return;
     }
+}
+
+template <typename T, int NTAPS>
+__device__ __forceinline__ void deposit(const SpreadArgs<T> &A, T *acc, int TT, int a, T e, T v,
+                                        int off) {
+    constexpr int Q0 = (NTAPS - 1) / 2;
     T w[NTAPS];
     tap_weights<T, NTAPS>(v * A.wscale, e, A.bi, A.e3, w);
-    const int rel = pr.a - off - Q0;  // cell of tap q = 0
+    const int rel = a - off - Q0;  // cell of tap q = 0
     if (rel >= 0 && rel + NTAPS <= TT) {
 #pragma unroll
         for (int q = 0; q < NTAPS; ++q) atomicAdd(&acc[rel + q], w[q]);
     } else if (A.wrap1) {
-        // single periodic tile (G <= tile): taps wrap around the grid
+        // single periodic tile (TT = G, a power of two): taps wrap around
 #pragma unroll
-        for (int q = 0; q < NTAPS; ++q) {
-            int cidx = (rel + q) % TT;
-            if (cidx < 0) cidx += TT;
-            atomicAdd(&acc[cidx], w[q]);
-        }
+        for (int q = 0; q < NTAPS; ++q) atomicAdd(&acc[(rel + q) & (TT - 1)], w[q]);
     } else {
 #pragma unroll
         for (int q = 0; q < NTAPS; ++q) {
             const int cidx = rel + q;
             if (cidx >= 0 && cidx < TT) atomicAdd(&acc[cidx], w[q]);
         }
+    }
+}
+
+// Points [begin, end) of the sorted order, seen from the tile whose first
+// cell is `off`.  Each thread takes U points per round and issues all their
+// loads before using any, so that the dependent gathers overlap.
+template <typename T, int NTAPS, int VM>
+__device__ __forceinline__ void spread_range(const SpreadArgs<T> &A, T *acc, int TT, int begin,
+                                             int end, int off, int c, int rep) {
+    constexpr int NT = 256, U = 4;
+    for (int s0 = begin + (int)threadIdx.x; s0 < end; s0 += NT * U) {
+        PointRec<T> pr[U];
+        int p[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int s = s0 + u * NT;
+            if (s < end) {
+                pr[u] = A.rec[s];
+                p[u] = A.perm[s];
+            }
+        }
+        T v[U], e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            e[u] = pr[u].e;
+            if (s0 + u * NT < end) point_value<T, VM>(A, p[u], c, rep, v[u], e[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (s0 + u * NT < end && (VM != VAL_COUNTS || v[u] != T(0)))
+                deposit<T, NTAPS>(A, acc, TT, pr[u].a, e[u], v[u], off);
     }
 }
 
@@ -417,8 +445,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
     T *acc = acc_all + (threadIdx.x >> 6) * TT;
     const int tstart = tile << A.tlog;
     // own tile, all classes
-    for (long s = bk[3 * tile] + threadIdx.x; s < bk[3 * tile + 3]; s += NT)
-        spread_point<T, NTAPS, VM>(A, acc, TT, s, tstart, c, rep);
+    spread_range<T, NTAPS, VM>(A, acc, TT, bk[3 * tile], bk[3 * tile + 3], tstart, c, rep);
     if (nt > 1) {
         int prev = tile - 1, next = tile + 1;
         bool hp = true, hn = true;
@@ -430,16 +457,12 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
             next = 0;
             hn = A.periodic;
         }
-        if (hp) {  // previous tile's right-edge class, seen from this tile
-            const int off = (prev << A.tlog) + TT;
-            for (long s = bk[3 * prev + 2] + threadIdx.x; s < bk[3 * prev + 3]; s += NT)
-                spread_point<T, NTAPS, VM>(A, acc, TT, s, off, c, rep);
-        }
-        if (hn) {  // next tile's left-edge class
-            const int off = (next << A.tlog) - TT;
-            for (long s = bk[3 * next] + threadIdx.x; s < bk[3 * next + 1]; s += NT)
-                spread_point<T, NTAPS, VM>(A, acc, TT, s, off, c, rep);
-        }
+        if (hp)  // previous tile's right-edge class, seen from this tile
+            spread_range<T, NTAPS, VM>(A, acc, TT, bk[3 * prev + 2], bk[3 * prev + 3],
+                                       (prev << A.tlog) + TT, c, rep);
+        if (hn)  // next tile's left-edge class
+            spread_range<T, NTAPS, VM>(A, acc, TT, bk[3 * next], bk[3 * next + 1],
+                                       (next << A.tlog) - TT, c, rep);
     }
     __syncthreads();
     T *g = A.grid + (long)slab * A.grid_stride + tstart;
