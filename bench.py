@@ -147,8 +147,6 @@ def bench_c2(args, torch, cu):
     x, y = clustered_points(N, 2024)
     xd = torch.from_numpy(x).cuda()
     yd = torch.from_numpy(y).cuda()
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plan = cu.NFFTPlan(xd, M)
@@ -156,30 +154,70 @@ def bench_c2(args, torch, cu):
     plan_ms = (time.perf_counter() - t0) * 1e3
     info = plan.info()
     out = torch.empty((1, M), dtype=torch.complex64, device="cuda")
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
     for _ in range(args.warmup):
-        plan.adjoint(yd, out)
-    torch.cuda.synchronize()
-
+        plan.adjoint(yd, out, stream=stream)
+    stream.synchronize()
     ev = HipEvents()
     K = args.steps
-    stage = ev.create(4 * K)
-    outer = ev.create(2)
     L = plan.L
+    outer = ev.create(2)
+    # host enqueue cost of one step (ctypes call + launches)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        plan.adjoint(yd, out, stream=stream)
+    host_step = (time.perf_counter() - t0) / 10
+    stream.synchronize()
+    # a spin kernel holds the stream while the host enqueues all K steps, so
+    # the GPU then runs them back to back; ms per spin cycle calibrated first
+    with torch.cuda.stream(stream):
+        ev.record(outer[0], sp)
+        torch.cuda._sleep(10_000_000)
+        ev.record(outer[1], sp)
+    stream.synchronize()
+    ms_per_cycle = ev.ms(outer[0], outer[1]) / 1e7
+    hold_ms = 1.5 * host_step * 1e3 * (K + 8) + 20.0
+    stage = ev.create(4 * K)
     torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        torch.cuda._sleep(int(hold_ms / ms_per_cycle))
     ev.record(outer[0], sp)
+    th = time.perf_counter()
     for k in range(K):
         arr = (ctypes.c_void_p * 4)(*stage[4 * k:4 * k + 4])
         L.cuna_nfft_set_profile_events(plan.h, arr, 4)
-        plan.adjoint(yd, out)
-    ev.record(outer[1], sp)
-    torch.cuda.synchronize()
+        plan.adjoint(yd, out, stream=stream)
     L.cuna_nfft_set_profile_events(plan.h, None, 0)
+    enqueue_ms = (time.perf_counter() - th) * 1e3
+    ev.record(outer[1], sp)
+    stream.synchronize()
+    torch.cuda.synchronize()
     total_ms = ev.ms(outer[0], outer[1])
     spread = np.mean([ev.ms(stage[4 * k], stage[4 * k + 1]) for k in range(K)])
     colp = np.mean([ev.ms(stage[4 * k + 1], stage[4 * k + 2]) for k in range(K)])
     rowp = np.mean([ev.ms(stage[4 * k + 2], stage[4 * k + 3]) for k in range(K)])
-    ev.destroy(stage + outer)
+    ev.destroy(stage)
     ms_step = total_ms / K
+    host_bound = enqueue_ms > hold_ms
+    # the same K transforms replayed from a captured graph (no events)
+    graph = cu.capture(lambda: [plan.adjoint(yd, out, stream=stream) for _ in range(K)], stream)
+    graph.launch(stream)
+    stream.synchronize()
+    ev.record(outer[0], sp)
+    graph.launch(stream)
+    ev.record(outer[1], sp)
+    stream.synchronize()
+    graph_ms = ev.ms(outer[0], outer[1]) / K
+    graph.close()
+    # eager launches from Python, for reference (host-bound at this size)
+    ev.record(outer[0], sp)
+    for _ in range(K):
+        plan.adjoint(yd, out, stream=stream)
+    ev.record(outer[1], sp)
+    stream.synchronize()
+    eager_ms = ev.ms(outer[0], outer[1]) / K
+    ev.destroy(outer)
 
     # accuracy of this output vs the FP64 direct NDFT on frequency bands
     got = out[0].cpu().numpy()
@@ -203,6 +241,9 @@ def bench_c2(args, torch, cu):
     return {
         "ms_per_step": ms_step,
         "value": 1e3 / ms_step,
+        "eager_ms_per_step": eager_ms,
+        "graph_ms_per_step": graph_ms,
+        "timing_host_bound": host_bound,
         "plan_ms": plan_ms,
         "rel_err_vs_ndft": err,
         "stages_ms": {k: v[0] for k, v in kernels.items()},
@@ -229,6 +270,205 @@ def bench_c2(args, torch, cu):
     }
 
 
+def bench_bootstrap(args, torch, cu):
+    """Batched bootstrap (the batching configuration): 1024 light curves of
+    log-uniform length in [100, 5e4] (8.2e6 points), times in days, FP64 phase
+    reduction with df = 1/3650 c/d, M = 2^14, 100 replicates per curve, two
+    transforms each (y at f, unit weights at 2f).  Batched = one persistent
+    fused launch replayed as a graph; plain = one unfused bootstrap call per
+    curve (spread + batched FFT kernels), all 1024 calls replayed as one graph."""
+    from tests._data import light_curves, rel_l2
+    import oracle
+
+    ncurves, M, nbs, seed, df = args.curves, 1 << 14, 100, 2024, 1.0 / 3650.0
+    t, y, offs = light_curves(ncurves, 3650)
+    td = torch.from_numpy(t).cuda()
+    yd = torch.from_numpy(y).cuda()
+    stream = torch.cuda.Stream()
+    ev = HipEvents()
+    sp = stream.cuda_stream
+    out_y = torch.empty((ncurves * nbs, M), dtype=torch.complex64, device="cuda")
+    out_w = torch.empty_like(out_y)
+
+    def timed(graph, reps):
+        e = ev.create(2)
+        graph.launch(stream)  # warm
+        stream.synchronize()
+        ev.record(e[0], sp)
+        for _ in range(reps):
+            graph.launch(stream)
+        ev.record(e[1], sp)
+        stream.synchronize()
+        ms = ev.ms(e[0], e[1]) / reps
+        ev.destroy(e)
+        return ms
+
+    plan_b = cu.NFFTPlan(t=td, df=df, offsets=offs, M=M, window2=True)
+    run_b = lambda: plan_b.bootstrap(yd, nbs, seed, window=True, out_y=out_y, out_w=out_w,
+                                     stream=stream)
+    run_b()
+    stream.synchronize()
+    g_b = cu.capture(run_b, stream)
+    ms_b = timed(g_b, args.boot_steps)
+    # accuracy of batched replicates vs the FP64 direct NDFT of the resampled points
+    xs = (df * t) % 1.0
+    errs = []
+    for c in (0, ncurves // 2, ncurves - 1):
+        n = int(offs[c + 1] - offs[c])
+        d = oracle.bootstrap_indices(seed, n, nbs) + offs[c]
+        for s in (0, nbs - 1):
+            ref = oracle.ndft(xs[d[s]], y.astype(np.float64)[d[s]], M)
+            errs.append(rel_l2(out_y[c * nbs + s].cpu().numpy(), ref))
+            refw = oracle.ndft(2 * xs[d[s]], None, M)
+            errs.append(rel_l2(out_w[c * nbs + s].cpu().numpy(), refw))
+    keep = out_y[:nbs].cpu().numpy()
+
+    plans = []
+    for c in range(ncurves):
+        sl = slice(int(offs[c]), int(offs[c + 1]))
+        plans.append(cu.NFFTPlan(t=td[sl], df=df, M=M, window2=True, unfused=True))
+    ys = [yd[int(offs[c]):int(offs[c + 1])] for c in range(ncurves)]
+
+    def run_p():
+        for c in range(ncurves):
+            plans[c].bootstrap(ys[c], nbs, seed, window=True,
+                               out_y=out_y[c * nbs:(c + 1) * nbs],
+                               out_w=out_w[c * nbs:(c + 1) * nbs], stream=stream)
+
+    run_p()
+    stream.synchronize()
+    agree = rel_l2(out_y[:nbs].cpu().numpy(), keep)
+    g_p = cu.capture(run_p, stream)
+    ms_p = timed(g_p, max(1, args.boot_steps // 2))
+    g_b.close()
+    g_p.close()
+    for p in plans:
+        p.close()
+    plan_b.close()
+    del out_y, out_w
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{ncurves} light curves, lengths log-uniform [100, 5e4] "
+                    f"({int(offs[-1])} points), M=2^14, {nbs} replicates x 2 transforms "
+                    "(y at f, unit weights at 2f), FP32 + FP64 phase reduction",
+        "lc_per_s_batched": ncurves / (ms_b * 1e-3),
+        "lc_per_s_plain": ncurves / (ms_p * 1e-3),
+        "ratio": ms_p / ms_b,
+        "ms_batched": ms_b,
+        "ms_plain": ms_p,
+        "max_rel_err_vs_ndft": max(errs),
+        "batched_vs_plain_rel_diff": agree,
+        "transforms_per_s_batched": ncurves * nbs * 2 / (ms_b * 1e-3),
+    }
+
+
+def bench_precision(torch, cu):
+    """Mixed precision on a long baseline: N = 2e5 times over 100 years (nightly
+    cadence, +-4 h jitter, 120-day seasonal gaps), M = 2^18, df = 1/(5 baseline).
+    FP64 phase reduction + FP32 gridding/FFT (libcunaf) and all-FP64 (libcunad)
+    against the FP64 direct NDFT, per frequency band."""
+    from tests._data import long_baseline_times, rel_l2
+    import oracle
+
+    n, M = 200_000, 1 << 18
+    t, y = long_baseline_times(n, 36500)
+    df = 1.0 / (5.0 * (t[-1] - t[0]))
+    td = torch.from_numpy(t).cuda()
+    p32 = cu.NFFTPlan(t=td, df=df, M=M, dtype=torch.float32)
+    p64 = cu.NFFTPlan(t=td, df=df, M=M, dtype=torch.float64)
+    F32 = p32.adjoint(torch.from_numpy(y).cuda())[0].cpu().numpy()
+    F64 = p64.adjoint(torch.from_numpy(y.astype(np.float64)).cuda())[0].cpu().numpy()
+    # FP32 sample positions (what an FP32 x loses at this baseline)
+    x32 = (((df * t) + 0.5) % 1.0 - 0.5).astype(np.float32)
+    Fn = cu.NFFTPlan(torch.from_numpy(x32).cuda(), M).adjoint(
+        torch.from_numpy(y).cuda())[0].cpu().numpy()
+    xs = (df * t) % 1.0
+    nb = 8
+    k0 = (np.linspace(-M // 2, M // 2 - 64, nb)).astype(np.int64)
+    ref = oracle.ndft_bands(xs, y.astype(np.float64), k0, 64)
+    bands = []
+    for b_, k in enumerate(k0):
+        sl = slice(int(k) + M // 2, int(k) + M // 2 + 64)
+        bands.append({"k0": int(k), "mixed": rel_l2(F32[sl], ref[b_]),
+                      "fp64": rel_l2(F64[sl], ref[b_]), "fp32_x": rel_l2(Fn[sl], ref[b_])})
+    p32.close(); p64.close()
+    return {
+        "workload": "N=2e5 times over 100 y (nightly cadence, +-4 h jitter, 120-d gaps), "
+                    "M=2^18, df=1/(5 baseline)",
+        "max_rel_err_mixed": max(b_["mixed"] for b_ in bands),
+        "max_rel_err_fp64": max(b_["fp64"] for b_ in bands),
+        "max_rel_err_fp32_positions": max(b_["fp32_x"] for b_ in bands),
+        "bands": bands,
+    }
+
+
+def bench_differentiation(torch, cu, steps=20):
+    """64 curves x N = 2e4, M = 2^16, L = sum_k w_k |F_k|^2: forward (adjoint)
+    and backward (d/dy and d/dx) times of the batch, each replayed as a graph."""
+    from tests._data import rel_l2
+    import oracle
+
+    B, n, M = 64, 20_000, 1 << 16
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-0.5, 0.5, (B, n)).astype(np.float32)
+    y = rng.standard_normal((B, n)).astype(np.float32)
+    w = rng.uniform(0, 1, M).astype(np.float32)
+    xd = torch.from_numpy(x.reshape(-1)).cuda()
+    yd = torch.from_numpy(y.reshape(-1)).cuda()
+    wd = torch.from_numpy(w).cuda()
+    offs = np.arange(B + 1, dtype=np.int32) * n
+    plan = cu.NFFTPlan(xd, M, offsets=offs)
+    stream = torch.cuda.Stream()
+    out = torch.empty((B, M), dtype=torch.complex64, device="cuda")
+    g = torch.empty_like(out)
+    dy = torch.empty(B * n, dtype=torch.float32, device="cuda")
+    dx = torch.empty_like(dy)
+    L = plan.L
+
+    def fwd():
+        plan.adjoint(yd, out, stream=stream)
+
+    def bwd():
+        st = L.cuna_nfft_backward(plan.h, yd.data_ptr(), g.data_ptr(), dy.data_ptr(),
+                                  dx.data_ptr(), stream.cuda_stream)
+        assert st == 0
+
+    fwd()
+    stream.synchronize()
+    g.copy_(2 * wd * out)
+    bwd()
+    stream.synchronize()
+    ev = HipEvents()
+    res = {}
+    for name, fn in (("forward", fwd), ("backward", bwd)):
+        gr = cu.capture(lambda: [fn() for _ in range(steps)], stream)
+        gr.launch(stream)
+        stream.synchronize()
+        e = ev.create(2)
+        ev.record(e[0], stream.cuda_stream)
+        gr.launch(stream)
+        ev.record(e[1], stream.cuda_stream)
+        stream.synchronize()
+        res[name] = ev.ms(e[0], e[1]) / steps
+        ev.destroy(e)
+        gr.close()
+    # gradient of curve 0 vs the analytic FP64 NDFT gradient
+    F0 = oracle.ndft(x[0].astype(np.float64), y[0].astype(np.float64), M)
+    ady, adx = oracle.ndft_grad(x[0].astype(np.float64), y[0].astype(np.float64), M,
+                                2 * w.astype(np.float64) * F0)
+    gy = dy[:n].cpu().numpy()
+    gx = dx[:n].cpu().numpy()
+    plan.close()
+    return {
+        "workload": "64 curves x N=2e4, x~U, y~N(0,1), M=2^16, L=sum w_k|F_k|^2, FP32",
+        "forward_ms": res["forward"],
+        "backward_ms": res["backward"],
+        "backward_over_forward": res["backward"] / res["forward"],
+        "grad_y_rel_err_vs_ndft": rel_l2(gy, ady),
+        "grad_x_rel_err_vs_ndft": rel_l2(gx, adx),
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +477,13 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-bootstrap", action="store_true")
+    ap.add_argument("--curves", type=int, default=1024)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the precision and differentiation workloads")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one device copy of exactly 256 MiB (a known-size kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--boot-steps", type=int, default=4)
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU kernel; "
@@ -251,6 +498,12 @@ def main(argv=None):
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     cu.load("f")
+    if args.calibrate_copy:
+        a = torch.empty(1 << 26, dtype=torch.float32, device="cuda")  # 256 MiB
+        c = torch.empty_like(a)
+        c.copy_(a)
+        torch.cuda.synchronize()
+        del a, c
     r = bench_c2(args, torch, cu)
     bw = copy_bandwidth(torch)
     res = {
@@ -275,12 +528,21 @@ def main(argv=None):
         },
         "rel_err_vs_ndft": r["rel_err_vs_ndft"],
         "plan_ms": r["plan_ms"],
+        "eager_ms_per_step": r["eager_ms_per_step"],
+        "graph_ms_per_step": r["graph_ms_per_step"],
+        "timing_host_bound": r["timing_host_bound"],
         "stages_ms": r["stages_ms"],
         "roofline": r["roofline"],
         "pipeline": r["pipeline"],
         "copy_bw_GBps": bw / 1e9,
         "cpu_baseline": None,
     }
+    if not args.no_bootstrap:
+        res["bootstrap"] = bench_bootstrap(args, torch, cu)
+    if not args.no_extras:
+        cu.load("d")
+        res["precision"] = bench_precision(torch, cu)
+        res["differentiation"] = bench_differentiation(torch, cu)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(r["x"], r["y"], r["M"], args.cpu_seconds)
     print(json.dumps(res))

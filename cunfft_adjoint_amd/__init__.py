@@ -8,6 +8,9 @@ from ._lib import CunaError, load, lib_path  # noqa: F401
 from .api import (  # noqa: F401
     CunaPlan,
     FilterProperties,
+    Graph,
+    autotune,
+    capture,
     NFFTPlan,
     cunfft_adjoint,
     cunfft_adjoint_raw,

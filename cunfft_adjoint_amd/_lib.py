@@ -26,6 +26,7 @@ CUNA_ERROR_NOT_FOUND = 5
 CUNA_MODE_ACCURATE = 0
 CUNA_MODE_COMPAT = 1
 CUNA_PLAN_WINDOW2 = 0x1
+CUNA_PLAN_UNFUSED = 0x2
 
 # FLAGS (inc/cuna_typedefs.h:110-116 of the reference)
 NO_FLAGS = 0x01
@@ -63,7 +64,7 @@ def make_types(prec: str):
 
 class NfftParams(ctypes.Structure):
     _fields_ = [("mode", c_int), ("sigma", c_int), ("m", c_int), ("tile_log2", c_int),
-                ("flags", c_int)]
+                ("flags", c_int), ("fft_w", c_int), ("fft_h", c_int)]
 
 
 class NfftInfo(ctypes.Structure):
@@ -102,12 +103,18 @@ def _protos(prec: str):
         "cuna_nfft_plan_create": (c_int, [P, P, c_int, c_int, P, P]),
         "cuna_nfft_plan_create_times": (c_int, [P, P, c_int, c_double, c_int, P, P]),
         "cuna_nfft_plan_create_batch": (c_int, [P, P, P, c_int, c_int, P, P]),
+        "cuna_nfft_plan_create_batch_times": (c_int, [P, P, P, c_int, c_double, c_int, P, P]),
         "cuna_nfft_adjoint": (c_int, [P, P, P, P]),
         "cuna_nfft_bootstrap": (c_int, [P, P, c_int, c_uint, P, P, P]),
         "cuna_nfft_backward": (c_int, [P, P, P, P, P, P]),
         "cuna_nfft_plan_info": (c_int, [P, P]),
         "cuna_nfft_set_profile_events": (c_int, [P, P, c_int]),
         "cuna_nfft_plan_destroy": (c_int, [P]),
+        "cuna_nfft_autotune": (c_int, [P, c_int, c_int, c_double, c_char_p, P, P]),
+        "cuna_stream_capture_begin": (c_int, [P]),
+        "cuna_stream_capture_end": (c_int, [P, P]),
+        "cuna_graph_launch": (c_int, [P, P]),
+        "cuna_graph_destroy": (c_int, [P]),
         "cuna_fft": (c_int, [P, c_int, c_int, c_int, P]),
     }
 

@@ -226,7 +226,8 @@ class NFFTPlan:
 
     def __init__(self, x=None, M: int = 0, *, t=None, df: float | None = None, offsets=None,
                  mode: str = "accurate", sigma: int = 2, m: int | None = None,
-                 tile_log2: int = 0, window2: bool = False, dtype=None, stream=None):
+                 tile_log2: int = 0, window2: bool = False, unfused: bool = False, dtype=None,
+                 fft_w: int = 0, fft_h: int = 0, stream=None):
         import torch
 
         if x is not None:
@@ -241,14 +242,22 @@ class NFFTPlan:
         if m is not None:
             p.m = m
         p.tile_log2 = tile_log2
-        p.flags = _lib.CUNA_PLAN_WINDOW2 if window2 else 0
+        p.fft_w, p.fft_h = fft_w, fft_h
+        p.flags = (_lib.CUNA_PLAN_WINDOW2 if window2 else 0) | (
+            _lib.CUNA_PLAN_UNFUSED if unfused else 0)
         h = ctypes.c_void_p()
         s = _stream(stream)
         if t is not None:
             if t.dtype != torch.float64:
                 raise TypeError("times must be float64")
-            st = self.L.cuna_nfft_plan_create_times(ctypes.byref(h), _ptr(t), t.numel(),
-                                                    float(df), M, ctypes.addressof(p), s)
+            if offsets is not None:
+                off = np.ascontiguousarray(offsets, dtype=np.int32)
+                st = self.L.cuna_nfft_plan_create_batch_times(
+                    ctypes.byref(h), _ptr(t), off.ctypes.data, len(off) - 1, float(df), M,
+                    ctypes.addressof(p), s)
+            else:
+                st = self.L.cuna_nfft_plan_create_times(ctypes.byref(h), _ptr(t), t.numel(),
+                                                        float(df), M, ctypes.addressof(p), s)
             self.n = t.numel()
         elif offsets is not None:
             off = np.ascontiguousarray(offsets, dtype=np.int32)
@@ -282,9 +291,14 @@ class NFFTPlan:
         check(self.L, self.L.cuna_nfft_adjoint(self.h, _ptr(y), _ptr(out), _stream(stream)))
         return out
 
-    def bootstrap(self, y, nbs: int, seed: int = 0, window: bool = False, stream=None):
-        out_y = self._out(self.ncurves * nbs)
-        out_w = self._out(self.ncurves * nbs) if window else None
+    def bootstrap(self, y, nbs: int, seed: int = 0, window: bool = False, stream=None,
+                  out_y=None, out_w=None):
+        if out_y is None:
+            out_y = self._out(self.ncurves * nbs)
+        if window and out_w is None:
+            out_w = self._out(self.ncurves * nbs)
+        if not window:
+            out_w = None
         check(self.L, self.L.cuna_nfft_bootstrap(self.h, _ptr(y), nbs, seed & 0xFFFFFFFF,
                                                   _ptr(out_y), _ptr(out_w), _stream(stream)))
         return (out_y, out_w) if window else out_y
@@ -312,6 +326,60 @@ class NFFTPlan:
             pass
 
 
+def autotune(x, M: int, tol: float, cache_path: str | None = None, stream=None) -> dict:
+    """cuna_nfft_autotune: plan parameters (sigma, m, tile_log2, fft_w, fft_h)
+    for points x meeting `tol` against the direct NDFT, fastest on this GPU.
+    Pass the result as keyword arguments to NFFTPlan."""
+    prec = _prec_of(x)
+    L = load(prec)
+    p = _lib.NfftParams()
+    cp = None if cache_path is None else cache_path.encode()
+    check(L, L.cuna_nfft_autotune(_ptr(x), x.numel(), M, float(tol), cp, ctypes.addressof(p),
+                                  _stream(stream)))
+    return {"sigma": p.sigma, "m": p.m, "tile_log2": p.tile_log2, "fft_w": p.fft_w,
+            "fft_h": p.fft_h}
+
+
+class Graph:
+    """A captured sequence of library calls (cuna_stream_capture_*)."""
+
+    def __init__(self, L, handle):
+        self.L, self.h = L, handle
+
+    def launch(self, stream=None):
+        check(self.L, self.L.cuna_graph_launch(self.h, _stream(stream)))
+
+    def close(self):
+        if self.h:
+            self.L.cuna_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def capture(fn, stream, prec: str = "f") -> Graph:
+    """Record the library calls made by fn() on `stream` (a torch.cuda.Stream,
+    not the null stream) into a graph.  Run fn() once before capturing so that
+    its workspaces exist, and pass preallocated outputs."""
+    L = load(prec)
+    s = _stream(stream)
+    check(L, L.cuna_stream_capture_begin(s))
+    g = ctypes.c_void_p()
+    try:
+        fn()
+    except BaseException:
+        L.cuna_stream_capture_end(s, ctypes.byref(g))
+        if g.value:
+            L.cuna_graph_destroy(g.value)
+        raise
+    check(L, L.cuna_stream_capture_end(s, ctypes.byref(g)))
+    return Graph(L, g.value)
+
+
 def fft(data, sign: int = +1, stream=None):
     """In-place unnormalised DFT along the last axis (power-of-two length)."""
     import torch
@@ -326,5 +394,6 @@ def fft(data, sign: int = +1, stream=None):
 __all__ = [
     "CunaError", "cunfft_adjoint", "FilterProperties", "generate_filter_properties",
     "generate_pinned_filter_properties", "cunfft_adjoint_raw", "cunfft_adjoint_raw_async",
-    "cunfft_adjoint_raw_async_bootstrap", "CunaPlan", "NFFTPlan", "fft",
+    "cunfft_adjoint_raw_async_bootstrap", "CunaPlan", "NFFTPlan", "fft", "Graph", "capture",
+    "autotune",
 ]

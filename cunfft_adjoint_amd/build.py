@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIBDIR = os.path.join(HERE, "lib")
 BUILDDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["cuna_kernels.hip", "cuna_api.hip"]
+SOURCES = ["cuna_kernels.hip", "cuna_api.hip", "cuna_tune.hip"]
 ARCH = os.environ.get("CUNA_OFFLOAD_ARCH", "gfx950")
 
 # packed-fp32 instructions are disabled for all device code

@@ -37,24 +37,6 @@ void clear_error() {
 bool abort_on_error() { return g_abort; }
 [[noreturn]] void fail(int code, const std::string &msg) { throw Error{code, msg}; }
 
-template <class F>
-static int guarded(F &&f) {
-    try {
-        clear_error();
-        f();
-        return CUNA_SUCCESS;
-    } catch (const Error &e) {
-        set_error(e.code, e.msg);
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        set_error(CUNA_ERROR_OUT_OF_MEMORY, "host allocation failed");
-        return CUNA_ERROR_OUT_OF_MEMORY;
-    } catch (...) {
-        set_error(CUNA_ERROR_HIP, "unexpected exception");
-        return CUNA_ERROR_HIP;
-    }
-}
-
 using R = dTyp;
 
 // ------------------------------------------------------------- helpers --
@@ -144,12 +126,12 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
     CUNA_HIP(hipMemcpyAsync(coff.ptr, coff_h.data(), sizeof(int) * coff_h.size(),
                             hipMemcpyHostToDevice, s));
     // temporaries, stream-ordered
-    DevBuf d_chunks, d_mbase, d_nch, keys, rec_tmp, hist, bsum;
+    DevBuf d_chunks, d_mbase, d_nch, keys, hist, bsum;
     d_chunks.alloc(sizeof(int4) * std::max<size_t>(chunks.size(), 1), s);
     d_mbase.alloc(sizeof(long) * ncurves, s);
     d_nch.alloc(sizeof(int) * ncurves, s);
     keys.alloc(sizeof(int) * std::max<long>(n, 1), s);
-    rec_tmp.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1), s);
+    rec_orig.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1));
     hist.alloc(sizeof(int) * std::max<long>(hist_len, 1), s);
     bsum.alloc(sizeof(int) * ceil_div(std::max<long>(hist_len, 1), kScanTile), s);
     if (!chunks.empty())
@@ -175,7 +157,7 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
     W.ncurves = ncurves;
     W.hist_len = hist_len;
     W.keys = keys.as<int>();
-    W.rec_tmp = rec_tmp.as<PointRec<T>>();
+    W.rec_tmp = rec_orig.as<PointRec<T>>();
     W.hist = hist.as<int>();
     W.bsum = bsum.as<int>();
     if (n > 0 || A.E3) {
@@ -189,6 +171,16 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
             launch_sort<T>(A, W, rec.as<PointRec<T>>(), perm.as<int>(), bucket.as<int>(), s);
         }
     }
+    // work-queue order of the curves for the fused batched kernels
+    std::vector<int> ord(ncurves);
+    for (int c = 0; c < ncurves; ++c) ord[c] = c;
+    std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
+        return offsets[u + 1] - offsets[u] > offsets[v + 1] - offsets[v];
+    });
+    order_h = std::move(ord);
+    order.alloc(sizeof(int) * ncurves);
+    CUNA_HIP(hipMemcpyAsync(order.ptr, order_h.data(), sizeof(int) * ncurves,
+                            hipMemcpyHostToDevice, s));
     // the host tables stay alive with the plan: the copies above are async
     chunks_h = std::move(chunks);
     mbase_h = std::move(mbase);
@@ -223,6 +215,8 @@ void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s, hipEv
     FftShape sh;
     sh.lgN = lgN;
     sh.batch = batch;
+    sh.w = spec.fft_w;
+    sh.h = spec.fft_h;
     if (spec.mode == CUNA_MODE_ACCURATE) {
         SinkAccurate<T> sk;
         sk.out = out;
@@ -279,14 +273,60 @@ void Plan<T>::bootstrap_compat(const T *y, cplx<T> *out, int nbs, uint32_t seed,
 }
 
 template <typename T>
+bool Plan<T>::bootstrap_fused_ok(bool window) const {
+    return !(spec.flags & CUNA_PLAN_UNFUSED) && boot_fused_supported<T>(lgN, ntaps) &&
+           (!window || has_pts2);
+}
+
+template <typename T>
+void Plan<T>::prepare_bootstrap(int nbs, bool window) {
+    if (bootstrap_fused_ok(window)) {
+        counter.ensure(sizeof(int));
+        return;
+    }
+    counts.ensure(sizeof(int) * (size_t)std::max<long>(n(), 1) * nbs);
+    grid.ensure(sizeof(T) * (size_t)G * ncurves() * nbs);
+}
+
+template <typename T>
 void Plan<T>::bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx<T> *out_w,
                         hipStream_t s) {
     require(spec.mode == CUNA_MODE_ACCURATE, "bootstrap needs an accurate-mode plan");
     require(nbs > 0, "nbs must be positive");
     require(out_w == nullptr || has_pts2, "window output needs a plan made with CUNA_PLAN_WINDOW2");
+    require(out_y != nullptr, "out_y is NULL");
     const int nc = ncurves();
+    prepare_bootstrap(nbs, out_w != nullptr);
+    if (bootstrap_fused_ok(out_w != nullptr)) {
+        BootArgs<T> B{};
+        B.rec_x = pts.rec_orig.template as<PointRec<T>>();
+        B.rec_2x = has_pts2 ? pts2.rec_orig.template as<PointRec<T>>() : nullptr;
+        B.y = y;
+        B.coff = pts.coff.template as<int>();
+        B.order = pts.order.template as<int>();
+        B.counter = counter.as<int>();
+        B.out_y = out_y;
+        B.out_w = out_w;
+        B.ncurves = nc;
+        B.nbs = nbs;
+        B.ntr = out_w ? 2 : 1;
+        const long total = (long)nc * nbs * B.ntr;
+        require(total < (1L << 31), "too many bootstrap items");
+        B.total = (int)total;
+        B.seed = seed;
+        B.bi = T(1) / beta;
+        B.wscale = wscale;
+        for (int q = 0; q < kMaxTaps; ++q) B.e3[q] = e3[q];
+        const double pi = 3.14159265358979323846;
+        B.sink.M = M;
+        B.sink.ostride = M;
+        B.sink.invG = T(1) / (T)G;
+        B.sink.coef = (T)(pi * pi * (double)beta);
+        B.sink.pref = (T)(1.0 / std::sqrt(pi * (double)beta));
+        launch_boot_fused<T>(B, lgN, ntaps, s);
+        return;
+    }
     const long n_ = n();
-    counts.ensure(sizeof(int) * (size_t)std::max<long>(n_, 1) * nbs);
     long tot = n_ * nbs;
     hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(std::max(tot, 1L), 256)), dim3(256), 0, s,
                        counts.as<int>(), tot, 0);
@@ -296,7 +336,6 @@ void Plan<T>::bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx
                            pts.coff.template as<int>(), nc, n_, nbs, seed, counts.as<int>());
         CUNA_LAUNCH_CHECK();
     }
-    grid.ensure(sizeof(T) * (size_t)G * nc * nbs);
     for (int pass = 0; pass < 2; ++pass) {
         cplx<T> *out = pass == 0 ? out_y : out_w;
         if (!out) continue;
@@ -336,6 +375,7 @@ void Plan<T>::backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s
     FftShape sh;
     sh.lgN = lgN;
     sh.batch = nc;
+    sh.w = spec.fft_w;
     fft_c2r_inverse<T>(sh, ld, work.as<cplx<T>>(), grid.as<T>(), s);
     GatherArgs<T> A{};
     A.rec = pts.rec.template as<PointRec<T>>();
@@ -494,6 +534,13 @@ std::unique_ptr<Plan<T>> make_plan_compat(const T *x, int n, int ng, T *E1, T *E
 }  // namespace cuna
 
 namespace cuna {
+
+// the plan machinery is also used by the autotuner's translation unit
+template struct TiledPoints<R>;
+template struct Plan<R>;
+template std::unique_ptr<Plan<R>> make_plan_accurate<R>(const R *, const double *, double,
+                                                        const std::vector<int> &, int,
+                                                        const ModeSpec &, hipStream_t);
 
 // ------------------------------------------------------------- registry --
 // Associates the spread plan with the device filter_properties pointer that
@@ -788,6 +835,10 @@ static ModeSpec spec_from(const cuna_nfft_params *params) {
     s.m = q.m;
     s.tile_log2 = q.tile_log2;
     s.flags = q.flags;
+    s.fft_w = q.fft_w;
+    s.fft_h = q.fft_h;
+    require(s.fft_w >= 0 && s.fft_w <= 8 && s.fft_h >= 0 && s.fft_h <= 4,
+            "fft_w must be in [0, 8] and fft_h in [0, 4]");
     require(s.mode == CUNA_MODE_ACCURATE || s.mode == CUNA_MODE_COMPAT, "unknown mode");
     require(s.tile_log2 == 0 || (s.tile_log2 >= 6 && s.tile_log2 <= 14), "tile_log2 out of range");
     return s;
@@ -800,6 +851,8 @@ CUNA_API void cuna_nfft_default_params(cuna_nfft_params *p) {
     p->m = sizeof(R) == 4 ? 8 : 13;
     p->tile_log2 = 0;
     p->flags = 0;
+    p->fft_w = 0;
+    p->fft_h = 0;
 }
 
 CUNA_API int cuna_nfft_plan_create(cuna_nfft_plan *plan, const dTyp *x, int n, int M,
@@ -863,6 +916,29 @@ CUNA_API int cuna_nfft_plan_create_batch(cuna_nfft_plan *plan, const dTyp *x, co
     });
 }
 
+CUNA_API int cuna_nfft_plan_create_batch_times(cuna_nfft_plan *plan, const double *t,
+                                               const int *offsets, int ncurves, double df, int M,
+                                               const cuna_nfft_params *params,
+                                               hipStream_t stream) {
+    return guarded([&] {
+        require(plan != nullptr && offsets != nullptr, "plan or offsets is NULL");
+        require(ncurves >= 1, "ncurves must be positive");
+        require(std::isfinite(df) && df > 0, "df must be positive");
+        ModeSpec spec = spec_from(params);
+        require(spec.mode == CUNA_MODE_ACCURATE, "batched plans are accurate-mode plans");
+        std::vector<int> off(offsets, offsets + ncurves + 1);
+        require(off.back() == 0 || t != nullptr, "t is NULL");
+        auto h = new cuna_nfft_plan_s;
+        try {
+            h->p = make_plan_accurate<R>(nullptr, t, df, off, M, spec, stream);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *plan = h;
+    });
+}
+
 CUNA_API int cuna_nfft_adjoint(cuna_nfft_plan plan, const dTyp *y, Complex *out,
                                hipStream_t stream) {
     return guarded([&] {
@@ -916,6 +992,54 @@ CUNA_API int cuna_nfft_set_profile_events(cuna_nfft_plan plan, hipEvent_t *event
 
 CUNA_API int cuna_nfft_plan_destroy(cuna_nfft_plan plan) {
     return guarded([&] { delete plan; });
+}
+
+// ------------------------------------------------------------- graphs --
+struct cuna_graph_s {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+};
+
+CUNA_API int cuna_stream_capture_begin(hipStream_t stream) {
+    return guarded([&] {
+        require(stream != nullptr, "capture needs a created stream (not the null stream)");
+        CUNA_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    });
+}
+
+CUNA_API int cuna_stream_capture_end(hipStream_t stream, cuna_graph *graph) {
+    return guarded([&] {
+        require(graph != nullptr, "graph is NULL");
+        auto G = new cuna_graph_s;
+        hipError_t e = hipStreamEndCapture(stream, &G->g);
+        if (e != hipSuccess) {
+            delete G;
+            hip_check(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        }
+        e = hipGraphInstantiate(&G->e, G->g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(G->g);
+            delete G;
+            hip_check(e, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        *graph = G;
+    });
+}
+
+CUNA_API int cuna_graph_launch(cuna_graph graph, hipStream_t stream) {
+    return guarded([&] {
+        require(graph && graph->e, "graph is NULL");
+        CUNA_HIP(hipGraphLaunch(graph->e, stream));
+    });
+}
+
+CUNA_API int cuna_graph_destroy(cuna_graph graph) {
+    return guarded([&] {
+        if (!graph) return;
+        if (graph->e) CUNA_HIP(hipGraphExecDestroy(graph->e));
+        if (graph->g) CUNA_HIP(hipGraphDestroy(graph->g));
+        delete graph;
+    });
 }
 
 CUNA_API int cuna_fft(Complex *data, int n, int batch, int sign, hipStream_t stream) {

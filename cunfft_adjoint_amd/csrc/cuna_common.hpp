@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <new>
 #include <string>
 
 #include "cuna_typedefs.h"
@@ -41,6 +42,26 @@ inline void hip_check(hipError_t e, const char *what, const char *file, int line
 
 inline void require(bool ok, const std::string &msg, int code = CUNA_ERROR_INVALID_VALUE) {
     if (!ok) fail(code, msg);
+}
+
+// Runs f, turning a cuna::Error (or any exception) into the recorded
+// thread-local status; returns that status.  Every extern "C" entry uses it.
+template <class F>
+int guarded(F &&f) {
+    try {
+        clear_error();
+        f();
+        return CUNA_SUCCESS;
+    } catch (const Error &e) {
+        set_error(e.code, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error(CUNA_ERROR_OUT_OF_MEMORY, "host allocation failed");
+        return CUNA_ERROR_OUT_OF_MEMORY;
+    } catch (...) {
+        set_error(CUNA_ERROR_HIP, "unexpected exception");
+        return CUNA_ERROR_HIP;
+    }
 }
 
 // --------------------------------------------------------------- complex --

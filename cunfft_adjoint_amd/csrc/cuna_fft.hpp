@@ -133,7 +133,8 @@ constexpr int pass_log2(int lg, int p) {
 // lds[e*W + w].  Pass 0 reads its inputs through `ld(e, w)`; every pass writes
 // LDS; the routine returns with natural-order results in LDS after a barrier.
 // tw[e] = exp(+2 pi i e / L) (conjugated for S = -1).
-template <typename T, int S, int LG, int W, int NT, int P, int LOGNS, class Loader>
+template <typename T, int S, int LG, int W, int NT, int P, int LOGNS, class Loader,
+          bool INPLACE = false>
 __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict__ tw,
                                          const Loader &ld) {
     constexpr int NP = num_passes(LG);
@@ -162,7 +163,9 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
                 }
             }
         }
-        if constexpr (P > 0) __syncthreads();
+        // pass 0 reads through the loader; when that reads this same LDS
+        // buffer (INPLACE) all reads must finish before the first write
+        if constexpr (P > 0 || INPLACE) __syncthreads();
 #pragma unroll
         for (int q = 0; q < BPT; ++q) {
             const int g = tid + q * NT;
@@ -184,7 +187,7 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
             }
         }
         __syncthreads();
-        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader>(lds, tw, ld);
+        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader, INPLACE>(lds, tw, ld);
     }
 }
 
@@ -192,6 +195,13 @@ template <typename T, int S, int LG, int W, int NT, class Loader>
 __device__ __forceinline__ void lds_fft(cplx<T> *lds, const cplx<T> *__restrict__ tw,
                                         const Loader &ld) {
     fft_pass<T, S, LG, W, NT, 0, 0, Loader>(lds, tw, ld);
+}
+
+// the data is already in `lds` (natural order); transformed in place
+template <typename T, int S, int LG, int NT>
+__device__ __forceinline__ void lds_fft_inplace(cplx<T> *lds, const cplx<T> *__restrict__ tw) {
+    auto fetch = [&](int e, int) { return lds[e]; };
+    fft_pass<T, S, LG, 1, NT, 0, 0, decltype(fetch), true>(lds, tw, fetch);
 }
 
 // ------------------------------------------------------ twiddle tables --

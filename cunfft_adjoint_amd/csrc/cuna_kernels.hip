@@ -7,11 +7,21 @@
 
 namespace cuna {
 
+// raise the dynamic LDS limit of a kernel once (not on every launch, so that
+// launches can be captured into a graph)
 template <typename K>
 static void set_lds(K kernel, size_t bytes) {
-    if (bytes > 64 * 1024)
-        CUNA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    if (bytes <= 64 * 1024) return;
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, size_t> done;
+    const void *key = reinterpret_cast<const void *>(kernel);
+    int dev = 0;
+    CUNA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto &have = done[{key, dev}];
+    if (have >= bytes) return;
+    CUNA_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    have = bytes;
 }
 
 static constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -101,16 +111,18 @@ static int env_int(const char *name) {
 }
 
 template <typename T>
-static int pick_seq(int lg, long nseq, int batch, int per_wg_mult, const char *env) {
-    int forced = env_int(env);
+static int pick_seq(int lg, long nseq, int batch, int per_wg_mult, const char *env,
+                    int requested = 0) {
+    int forced = requested ? requested : env_int(env);
+    auto valid = [&](int w) {
+        long lds = (long)w * per_wg_mult * (1L << lg) * (long)sizeof(cplx<T>);
+        return w >= 1 && w <= 8 && (w & (w - 1)) == 0 && lds <= 65536 &&
+               nseq % (w * per_wg_mult) == 0;
+    };
+    if (forced && valid(forced)) return forced;
     int best = 1;
     for (int w = 8; w >= 1; w >>= 1) {
-        long lds = (long)w * per_wg_mult * (1L << lg) * (long)sizeof(cplx<T>);
-        if (lds > 65536 || nseq % (w * per_wg_mult) != 0) continue;
-        if (forced) {
-            if (w == forced) return w;
-            continue;
-        }
+        if (!valid(w)) continue;
         if ((nseq / (w * per_wg_mult)) * (long)batch >= 512) return w;
         best = std::min(best, w);
     }
@@ -171,8 +183,8 @@ static void real_forward_impl(const FftShape &sh, T *grid, const Sink &sk, hipSt
     }
     const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
     cplx<T> *z = reinterpret_cast<cplx<T> *>(grid);
-    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W");
-    const int H = std::min(4, pick_seq<T>(lg1, 1L << lg2, sh.batch, 2, "CUNA_FFT_H"));
+    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W", sh.w);
+    const int H = std::min(4, pick_seq<T>(lg1, 1L << lg2, sh.batch, 2, "CUNA_FFT_H", sh.h));
     dispatch_lg<7, 12>(lg2, [&](auto c) {
         constexpr int LG2 = decltype(c)::value;
         dispatch_w<T, LG2>(W, [&](auto w) {
@@ -216,8 +228,8 @@ void fft_c2r_inverse(const FftShape &sh, const LoadHermitian<T> &ld, cplx<T> *wo
         return;
     }
     const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
-    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W");
-    const int H = pick_seq<T>(lg1, 1L << lg2, sh.batch, 1, "CUNA_FFT_H");
+    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W", sh.w);
+    const int H = pick_seq<T>(lg1, 1L << lg2, sh.batch, 1, "CUNA_FFT_H", sh.h);
     dispatch_lg<7, 12>(lg2, [&](auto c) {
         constexpr int LG2 = decltype(c)::value;
         dispatch_w<T, LG2>(W, [&](auto w) {
@@ -253,8 +265,8 @@ void fft_complex(const FftShape &sh, cplx<T> *data, cplx<T> *work, int sign, hip
         return;
     }
     const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
-    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W");
-    const int H = pick_seq<T>(lg1, 1L << lg2, sh.batch, 1, "CUNA_FFT_H");
+    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W", sh.w);
+    const int H = pick_seq<T>(lg1, 1L << lg2, sh.batch, 1, "CUNA_FFT_H", sh.h);
     dispatch_lg<7, 12>(lg2, [&](auto c) {
         constexpr int LG2 = decltype(c)::value;
         dispatch_w<T, LG2>(W, [&](auto w) {
@@ -391,6 +403,54 @@ void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s)
     });
 }
 
+// ------------------------------------------------------ fused bootstrap --
+// grids up to 2^15 cells (float) / 2^14 (double): 128 KB of LDS, so one
+// persistent workgroup of 1024 threads per CU
+template <typename T>
+bool boot_fused_supported(int lgN, int ntaps) {
+    return lgN >= 6 && lgN <= max_small_lg<T>() && (ntaps == 12 || ntaps == 16 || ntaps == 20 ||
+                                                    ntaps == 24 || ntaps == 26 || ntaps == 32);
+}
+
+template <typename T, int LG, int NTAPS>
+static void boot_fused_one(BootArgs<T> A, hipStream_t s) {
+    constexpr int NT = fft_threads(LG, 1) < 256 ? 256 : fft_threads(LG, 1);
+    auto &tc = TwiddleCache<T>::get();
+    A.tw = tc.fine(LG);
+    A.twG = tc.two_level(LG + 1);
+    const size_t lds = sizeof(cplx<T>) << LG;
+    auto k = k_boot_fused<T, LG, NTAPS, NT>;
+    set_lds(k, lds);
+    int dev = 0, ncu = 0;
+    CUNA_HIP(hipGetDevice(&dev));
+    CUNA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int per_cu = 0;
+    CUNA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k),
+                                                          NT, lds));
+    const int nblk = std::max(1, std::min(A.total, ncu * std::max(per_cu, 1)));
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, A.counter);
+    CUNA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, A);
+    CUNA_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_boot_fused(BootArgs<T> A, int lgN, int ntaps, hipStream_t s) {
+    require(boot_fused_supported<T>(lgN, ntaps), "no fused bootstrap kernel for this size",
+            CUNA_ERROR_UNSUPPORTED);
+    dispatch_lg<6, max_small_lg<T>()>(lgN, [&](auto c) {
+        constexpr int LG = decltype(c)::value;
+        switch (ntaps) {
+            case 12: boot_fused_one<T, LG, 12>(A, s); break;
+            case 16: boot_fused_one<T, LG, 16>(A, s); break;
+            case 20: boot_fused_one<T, LG, 20>(A, s); break;
+            case 24: boot_fused_one<T, LG, 24>(A, s); break;
+            case 26: boot_fused_one<T, LG, 26>(A, s); break;
+            default: boot_fused_one<T, LG, 32>(A, s); break;
+        }
+    });
+}
+
 // ------------------------------------------------------- instantiation --
 template void fft_real_forward<dTyp>(const FftShape &, dTyp *, const SinkAccurate<dTyp> &,
                                      hipStream_t, hipEvent_t);
@@ -403,5 +463,7 @@ template void launch_sort<dTyp>(SetupArgs<dTyp>, const SortWork<dTyp> &, PointRe
                                 int *, hipStream_t);
 template void launch_spread<dTyp>(const SpreadArgs<dTyp> &, int, int, int, hipStream_t);
 template void launch_gather<dTyp>(const GatherArgs<dTyp> &, int, int, hipStream_t);
+template bool boot_fused_supported<dTyp>(int, int);
+template void launch_boot_fused<dTyp>(BootArgs<dTyp>, int, int, hipStream_t);
 
 }  // namespace cuna

@@ -12,6 +12,8 @@ namespace cuna {
 struct FftShape {
     int lgN = 0;
     int batch = 1;
+    int w = 0;  // columns per workgroup in pass 1 (0: heuristic)
+    int h = 0;  // rows (real: row pairs) per workgroup in pass 2 (0: heuristic)
 };
 
 template <typename T>
@@ -66,6 +68,13 @@ template <typename T>
 void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStream_t s);
 template <typename T>
 void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s);
+
+// fused batched bootstrap (grid in LDS): false when this size/filter has no
+// fused kernel
+template <typename T>
+bool boot_fused_supported(int lgN, int ntaps);
+template <typename T>
+void launch_boot_fused(BootArgs<T> A, int lgN, int ntaps, hipStream_t s);
 
 // bytes of dynamic LDS a launch may use on gfx950
 constexpr size_t kMaxLds = 160 * 1024;

@@ -47,12 +47,15 @@ struct TiledPoints {
     std::vector<int4> chunks_h;
     std::vector<long> mbase_h;
     std::vector<int> nch_h;
+    std::vector<int> order_h;
     int ncurves = 0;
     long n = 0;
     std::vector<int> coff_h;
     int G = 0, tlog = 0, ntiles = 0, Kc = 0, olo = 0, ntaps = 0;
     bool periodic = true;
     DevMem coff, rec, perm, bucket, eps_orig;
+    DevMem rec_orig;  // (anchor, e) by original index
+    DevMem order;     // curves by decreasing length (work-queue order)
 
     // src: setup arguments with x/t/mode/compat fields filled
     void build(SetupArgs<T> A, const std::vector<int> &offsets, int G_, int tlog_, int ntaps_,
@@ -62,6 +65,7 @@ struct TiledPoints {
 struct ModeSpec {
     int mode = 0;  // CUNA_MODE_*
     int sigma = 2, m = 8, tile_log2 = 0, flags = 0;
+    int fft_w = 0, fft_h = 0;  // FFT workgroup shape (0: heuristic)
 };
 
 template <typename T>
@@ -78,7 +82,7 @@ struct Plan {
     const void *x_key = nullptr;
     TiledPoints<T> pts, pts2;  // pts2: points 2x (window at 2f)
     bool has_pts2 = false;
-    DevMem grid, work, counts;
+    DevMem grid, work, counts, counter;
     // optional profiling events, recorded on the execution stream before the
     // first kernel of adjoint() and after each of its kernels
     hipEvent_t prof[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -94,6 +98,10 @@ struct Plan {
     void bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx<T> *out_w,
                    hipStream_t s);
     void backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s);
+    // allocate every workspace a later call with these sizes needs (so that
+    // the call can be captured into a graph without allocating)
+    void prepare_bootstrap(int nbs, bool window);
+    bool bootstrap_fused_ok(bool window) const;
 
     SpreadArgs<T> spread_args(const TiledPoints<T> &P) const;
     void run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s,

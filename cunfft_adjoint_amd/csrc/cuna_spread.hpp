@@ -126,12 +126,12 @@ __device__ __forceinline__ int tile_key(int a, int tlog, int olo, int ohi, int G
     return 3 * tile + cls;
 }
 
-// one wave per chunk; 4 chunks per 256-thread block; LDS counters per wave
+// one wave per chunk, blockDim/64 chunks per block; LDS counters per wave
 template <typename T>
 __global__ __launch_bounds__(256) void k_setup_hist(SetupArgs<T> A) {
     int *cnt_all = reinterpret_cast<int *>(cuna_smem);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ch = blockIdx.x * 4 + wave;
+    const int ch = blockIdx.x * (blockDim.x >> 6) + wave;
     int *cnt = cnt_all + wave * A.Kc;
     for (int k = lane; k < A.Kc; k += 64) cnt[k] = 0;
     __syncthreads();
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs S, const PointRec<T
                                                  PointRec<T> *rec, int *perm) {
     int *cur_all = reinterpret_cast<int *>(cuna_smem);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ch = blockIdx.x * 4 + wave;
+    const int ch = blockIdx.x * (blockDim.x >> 6) + wave;
     int *cur = cur_all + wave * S.Kc;
     int4 cd = make_int4(0, 0, 0, 0);
     if (ch < S.nchunks) {
@@ -472,6 +472,88 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
 #pragma unroll
         for (int w = 1; w < NW; ++w) s += acc_all[w * TT + i];
         g[i] = s;
+    }
+}
+
+// ------------------------------------------- fused batched bootstrap --
+// One persistent workgroup per CU drains a queue of (curve, replicate,
+// transform) items.  An item draws the replicate's points
+// d = c0 + xorshift(wang(seed + rep*n_c + p)) % n_c, p < n_c, spreads them
+// into a grid of G = 2N cells held entirely in LDS, transforms it in place
+// (real input as N complex) and writes the deconvolved spectrum:
+// transform 0 is y at the points x_d, transform 1 the unit-weight window at 2 x_d.
+template <typename T>
+struct BootArgs {
+    const PointRec<T> *rec_x;   // (anchor, frac) of x by original index
+    const PointRec<T> *rec_2x;  // of 2x (window), nullable
+    const T *y;                 // nullable: unit weights
+    const int *coff;
+    const int *order;           // curves, longest first
+    int *counter;               // work-queue head, zeroed before the launch
+    cplx<T> *out_y, *out_w;
+    int ncurves, nbs, ntr, total;
+    uint32_t seed;
+    T bi, wscale;
+    T e3[kMaxTaps];
+    SinkAccurate<T> sink;       // out pointer set per item
+    const cplx<T> *tw;
+    Tw2<T> twG;
+};
+
+static __global__ void k_reset_counter(int *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+}
+
+template <typename T, int LG, int NTAPS, int NT>
+__global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
+    constexpr int N = 1 << LG, G = 2 * N, Q0 = (NTAPS - 1) / 2, U = 4;
+    T *grid = reinterpret_cast<T *>(cuna_smem);
+    cplx<T> *z = reinterpret_cast<cplx<T> *>(cuna_smem);
+    __shared__ int item_sh;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = atomicAdd(A.counter, 1);
+        for (int i = threadIdx.x; i < G; i += NT) grid[i] = T(0);
+        __syncthreads();
+        const int item = item_sh;
+        if (item >= A.total) break;  // uniform over the workgroup
+        const int per_curve = A.nbs * A.ntr;
+        const int ci = A.order[item / per_curve];
+        const int rem = item % per_curve;
+        const int rep = rem / A.ntr, tr = rem % A.ntr;
+        const int c0 = A.coff[ci];
+        const int nc = A.coff[ci + 1] - c0;
+        const PointRec<T> *P = tr == 0 ? A.rec_x : A.rec_2x;
+        for (int p0 = threadIdx.x; p0 < nc; p0 += NT * U) {
+            PointRec<T> pr[U];
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int p = p0 + u * NT;
+                if (p < nc) {
+                    const uint32_t t = (uint32_t)rep * (uint32_t)nc + (uint32_t)p;
+                    const int d = c0 + (int)(xorshift32(wang_hash(A.seed + t)) % (uint32_t)nc);
+                    pr[u] = P[d];
+                    v[u] = (tr == 0 && A.y) ? A.y[d] : T(1);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (p0 + u * NT < nc) {
+                    T w[NTAPS];
+                    tap_weights<T, NTAPS>(v[u] * A.wscale, pr[u].e, A.bi, A.e3, w);
+                    const int base = pr[u].a - Q0;
+#pragma unroll
+                    for (int q = 0; q < NTAPS; ++q) atomicAdd(&grid[(base + q) & (G - 1)], w[q]);
+                }
+            }
+        }
+        __syncthreads();
+        lds_fft_inplace<T, 1, LG, NT>(z, A.tw);
+        SinkAccurate<T> sk = A.sink;
+        cplx<T> *o = (tr == 0 ? A.out_y : A.out_w) + ((long)ci * A.nbs + rep) * sk.M;
+        for (int r = threadIdx.x; r <= N / 2; r += NT)
+            emit_pair(sk, o, (long)r, (long)N, z[r], z[(N - r) & (N - 1)], A.twG(r));
+        __syncthreads();
     }
 }
 

@@ -35,6 +35,8 @@ typedef struct cuna_nfft_plan_s *cuna_nfft_plan;
 
 /* plan flags */
 #define CUNA_PLAN_WINDOW2 0x1 /* also bin the points 2x (window transform at 2f) */
+#define CUNA_PLAN_UNFUSED 0x2 /* bootstrap through spread + batched FFT kernels instead of
+                                 the fused single-launch kernel (grids <= 2^15 cells) */
 
 typedef struct {
     int mode;      /* CUNA_MODE_ACCURATE (default) or CUNA_MODE_COMPAT */
@@ -42,6 +44,8 @@ typedef struct {
     int m;         /* accurate: filter half-width, 2m taps, 2..16 (default 8 / 13) */
     int tile_log2; /* spread tile of 2^tile_log2 cells; 0 = library default */
     int flags;     /* CUNA_PLAN_* */
+    int fft_w;     /* FFT pass 1: columns per workgroup (1,2,4,8; 0 = library choice) */
+    int fft_h;     /* FFT pass 2: rows (row pairs) per workgroup (1,2,4; 0 = library choice) */
 } cuna_nfft_params;
 
 typedef struct {
@@ -69,6 +73,12 @@ int cuna_nfft_plan_create_batch(cuna_nfft_plan *plan, const dTyp *x, const int *
                                 int ncurves, int M, const cuna_nfft_params *params,
                                 hipStream_t stream);
 
+/* as above from double times t (device): every curve on the frequency grid
+ * k*df, x_j = frac(df * t_j) evaluated in FP64 */
+int cuna_nfft_plan_create_batch_times(cuna_nfft_plan *plan, const double *t, const int *offsets,
+                                      int ncurves, double df, int M,
+                                      const cuna_nfft_params *params, hipStream_t stream);
+
 /* out: ncurves x M complex (curve-major).  y: n values or NULL (unit weights). */
 int cuna_nfft_adjoint(cuna_nfft_plan plan, const dTyp *y, Complex *out, hipStream_t stream);
 
@@ -94,6 +104,28 @@ int cuna_nfft_plan_info(cuna_nfft_plan plan, cuna_nfft_info *info);
  * execution stream.  nevents = 0 clears them. */
 int cuna_nfft_set_profile_events(cuna_nfft_plan plan, hipEvent_t *events, int nevents);
 int cuna_nfft_plan_destroy(cuna_nfft_plan plan);
+
+/* Autotuner: fills *params for accurate-mode plans of these points.  Per
+ * sigma in {2, 4} it takes the smallest m whose Gaussian-gridding error bound
+ * exp(-pi m (1 - 1/(2 sigma - 1))) plus the arithmetic's rounding floor meets
+ * tol (relative L2 error vs the direct NDFT), then times the spread tile
+ * (2^8..2^11 cells) and FFT workgroup shapes on the device and keeps the
+ * fastest.  Results are cached (appended) in a text file keyed by GPU,
+ * precision, n rounded to a power of two, M and tol: cache_path NULL uses
+ * $CUNA_TUNE_CACHE or ~/.cache/cuna_tune.txt, "" disables the cache. */
+int cuna_nfft_autotune(const dTyp *x, int n, int M, double tol, const char *cache_path,
+                       cuna_nfft_params *params, hipStream_t stream);
+
+/* Graphs: every call above made between cuna_stream_capture_begin and
+ * cuna_stream_capture_end on `stream` (a created stream, not the null
+ * stream) is recorded instead of run; cuna_graph_launch replays the sequence.
+ * Run each captured call once beforehand so that its workspaces exist: a
+ * capture allocates nothing.  The captured kernels use no memset nodes. */
+typedef struct cuna_graph_s *cuna_graph;
+int cuna_stream_capture_begin(hipStream_t stream);
+int cuna_stream_capture_end(hipStream_t stream, cuna_graph *graph);
+int cuna_graph_launch(cuna_graph graph, hipStream_t stream);
+int cuna_graph_destroy(cuna_graph graph);
 
 /* The FFT stage alone: in-place unnormalised DFT of `batch` sequences of n
  * (power of two) values, data[r] = sum_g data[g] exp(sign*2 pi i g r/n).

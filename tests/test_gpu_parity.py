@@ -314,3 +314,127 @@ def test_autograd_batched_loss():
         ady, adx = oracle.ndft_grad(xh[b], yh[b], M, g)
         assert rel_l2(y.grad[b].cpu().numpy(), ady) < 1e-10
         assert rel_l2(x.grad[b].cpu().numpy(), adx) < 1e-10
+
+
+# ------------------------------------------- batched bootstrap + graphs --
+def test_fused_batched_bootstrap_vs_unfused_and_oracle():
+    from tests._data import light_curves
+    t, y, offs = light_curves(24, 5, nmin=50, nmax=3000)
+    df, M, nbs, seed = 1.0 / 3650.0, 1 << 12, 6, 31
+    td, yd = dev(t), dev(y)
+    fused = cu.NFFTPlan(t=td, df=df, offsets=offs, M=M, window2=True)
+    unf = cu.NFFTPlan(t=td, df=df, offsets=offs, M=M, window2=True, unfused=True)
+    fy, fw = fused.bootstrap(yd, nbs, seed, window=True)
+    uy, uw = unf.bootstrap(yd, nbs, seed, window=True)
+    fy, fw, uy, uw = host(fy), host(fw), host(uy), host(uw)
+    assert rel_l2(fy, uy) < 1e-6 and rel_l2(fw, uw) < 1e-6
+    xs = (df * t) % 1.0
+    for c in (0, 7, 23):
+        n = offs[c + 1] - offs[c]
+        d = oracle.bootstrap_indices(seed, n, nbs) + offs[c]
+        for s in (0, nbs - 1):
+            ref = oracle.ndft(xs[d[s]], y.astype(np.float64)[d[s]], M)
+            assert rel_l2(fy[c * nbs + s], ref) < 1e-5
+            refw = oracle.ndft(2 * xs[d[s]], None, M)
+            assert rel_l2(fw[c * nbs + s], refw) < 1e-5
+    # per-curve plans give the same replicates as the batch
+    c = 7
+    sl = slice(offs[c], offs[c + 1])
+    one = cu.NFFTPlan(t=dev(t[sl]), df=df, M=M, window2=True)
+    oy = host(one.bootstrap(dev(y[sl]), nbs, seed))
+    assert rel_l2(oy, fy[c * nbs:(c + 1) * nbs]) < 1e-6
+
+
+def test_graph_replays_match_eager():
+    stream = torch.cuda.Stream()
+    x, y = clustered_points(100_000, 9)
+    M = 1 << 16
+    plan = cu.NFFTPlan(dev(x), M)
+    yd = dev(y)
+    out = torch.empty((1, M), dtype=torch.complex64, device="cuda")
+    with torch.cuda.stream(stream):
+        plan.adjoint(yd, out, stream=stream)
+    stream.synchronize()
+    eager = host(out).copy()
+    g = cu.capture(lambda: plan.adjoint(yd, out, stream=stream), stream)
+    for rep in range(4):
+        out.zero_()
+        torch.cuda.synchronize()
+        g.launch(stream)
+        stream.synchronize()
+        assert np.array_equal(host(out).view(np.uint32), eager.view(np.uint32)), rep
+        # eager library work between replays
+        plan.adjoint(yd * 2, out)
+        torch.cuda.synchronize()
+    g.close()
+    # the fused bootstrap (work-queue counter reset by a kernel node) replays too
+    from tests._data import light_curves
+    t, yy, offs = light_curves(16, 6, nmin=50, nmax=2000)
+    bp = cu.NFFTPlan(t=dev(t), df=1 / 3650.0, offsets=offs, M=1 << 12, window2=True)
+    ty = dev(yy)
+    oy = torch.empty((16 * 4, 1 << 12), dtype=torch.complex64, device="cuda")
+    ow = torch.empty_like(oy)
+    bp.bootstrap(ty, 4, 5, window=True, out_y=oy, out_w=ow, stream=stream)
+    stream.synchronize()
+    ey, ew = host(oy).copy(), host(ow).copy()
+    g = cu.capture(lambda: bp.bootstrap(ty, 4, 5, window=True, out_y=oy, out_w=ow,
+                                        stream=stream), stream)
+    for rep in range(3):
+        oy.zero_()
+        ow.zero_()
+        torch.cuda.synchronize()
+        g.launch(stream)
+        stream.synchronize()
+        assert rel_l2(host(oy), ey) < 1e-6 and rel_l2(host(ow), ew) < 1e-6, rep
+        bp.bootstrap(ty, 4, 9, window=True)
+        torch.cuda.synchronize()
+    g.close()
+
+
+# ------------------------------------------------------------ autotuner --
+def test_autotune_meets_tolerance_and_caches(tmp_path):
+    import time
+    n, M = 50_000, 1 << 16
+    x, y = uniform_points(n, 17)
+    xd, yd = dev(x), dev(y)
+    cache = str(tmp_path / "tune.txt")
+    ref = oracle.ndft_bands(x.astype(np.float64), y.astype(np.float64),
+                            np.array([-M // 2, -100, 5000], dtype=np.int64), 64)
+    for tol in (1e-3, 1e-5):
+        p = cu.autotune(xd, M, tol, cache_path=cache)
+        assert p["sigma"] in (2, 4) and 2 <= p["m"] <= 16
+        plan = cu.NFFTPlan(xd, M, **p)
+        got = host(plan.adjoint(yd))[0]
+        sel = np.stack([got[k + M // 2:k + M // 2 + 64] for k in (-M // 2, -100, 5000)])
+        assert rel_l2(sel, ref) < tol
+    t0 = time.perf_counter()
+    p2 = cu.autotune(xd, M, 1e-5, cache_path=cache)
+    assert time.perf_counter() - t0 < 0.5  # served from the cache file
+    assert p2 == p
+    assert len(open(cache).read().splitlines()) == 2
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_sigma4_and_explicit_fft_shapes(prec):
+    n, M = 20_000, 1 << 15
+    x, y = uniform_points(n, 41, NP[prec])
+    ref = oracle.ndft(x.astype(np.float64), y.astype(np.float64), M)
+    for kw in ({"sigma": 4, "m": 6 if prec == "f" else 10}, {"fft_w": 1, "fft_h": 4},
+               {"fft_w": 8, "fft_h": 1}, {"fft_w": 4, "fft_h": 2}):
+        plan = cu.NFFTPlan(dev(x), M, **kw)
+        got = host(plan.adjoint(dev(y)))[0]
+        assert rel_l2(got, ref) < (1e-5 if prec == "f" else 1e-10), kw
+
+
+def test_many_tiles_one_wave_per_sort_block():
+    # 8192 tiles: the sort's per-wave key counters (3 x 8192 ints) allow one
+    # wave per block; every chunk must still be binned
+    n, M = 300_000, 1 << 20
+    x, y = clustered_points(n, 77)
+    for tl in (8, 9):
+        plan = cu.NFFTPlan(dev(x), M, tile_log2=tl)
+        got = host(plan.adjoint(dev(y)))[0]
+        k0 = np.array([-M // 2, -20, 12345], dtype=np.int64)
+        ref = oracle.ndft_bands(x.astype(np.float64), y.astype(np.float64), k0, 64)
+        sel = np.stack([got[k + M // 2:k + M // 2 + 64] for k in k0])
+        assert rel_l2(sel, ref) < 1e-5, tl
