@@ -83,41 +83,42 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
     ntaps = ntaps_;
     olo = -((ntaps - 1) / 2);
     periodic = periodic_;
-    // tile: as requested, no larger than the grid, and few enough tiles that
-    // the per-curve key counters fit in LDS
-    // 4 private LDS copies of a tile must fit in 128 KB
-    const int tmax = sizeof(T) == 4 ? 13 : 12;
+    // tile: as requested, no larger than the grid, few enough tiles for the
+    // sort's per-wave key counters
+    const int tmax = 13;
     tlog = std::min(std::min(tlog_, tmax), ilog2(G));
     while ((G >> tlog) > 8192) tlog++;
     require(tlog <= tmax, "grid too large for the spread tiles", CUNA_ERROR_UNSUPPORTED);
-    ntiles = G >> tlog;
+    ntiles = (int)ceil_div(G, 1L << tlog);
     if (ntiles > 1 && (1 << tlog) < 2 * ntaps) {
         tlog = ilog2(G);
         ntiles = 1;
     }
     Kc = 3 * ntiles;
+    const int K1 = 1 << tlog;
 
     // chunk table: chunks of at most 2048 points, never crossing a curve
     constexpr int CH = 2048;
     std::vector<int4> chunks;
-    std::vector<long> mbase(ncurves);
-    std::vector<int> nch(ncurves);
-    long hist_len = 0;
+    std::vector<int> base(ncurves), nch(ncurves);
+    int total_chunks = 0;
     for (int c = 0; c < ncurves; ++c) {
         int nc = offsets[c + 1] - offsets[c];
         require(nc >= 0, "curve offsets must be non-decreasing");
         int k = (int)ceil_div(nc, CH);
         nch[c] = k;
-        mbase[c] = hist_len;
+        base[c] = total_chunks;
         for (int q = 0; q < k; ++q) {
             int st = offsets[c] + q * CH;
             chunks.push_back(make_int4(c, st, std::min(CH, offsets[c + 1] - st), q));
         }
-        hist_len += (long)Kc * k;
+        total_chunks += k;
     }
+    const long hist_len = (long)std::max(K1, Kc) * std::max(total_chunks, 1);
     coff.alloc(sizeof(int) * offsets.size());
     rec.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1));
     perm.alloc(sizeof(int) * std::max<long>(n, 1));
+    rec_orig.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1));
     bucket.alloc(sizeof(int) * (size_t)ncurves * (Kc + 1));
     if (keep_eps) {
         eps_orig.alloc(sizeof(T) * std::max<long>(n, 1));
@@ -126,50 +127,53 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
     CUNA_HIP(hipMemcpyAsync(coff.ptr, coff_h.data(), sizeof(int) * coff_h.size(),
                             hipMemcpyHostToDevice, s));
     // temporaries, stream-ordered
-    DevBuf d_chunks, d_mbase, d_nch, keys, hist, bsum;
+    DevBuf d_chunks, d_base, d_nch, keys, recA, permA, hist, bsum;
     d_chunks.alloc(sizeof(int4) * std::max<size_t>(chunks.size(), 1), s);
-    d_mbase.alloc(sizeof(long) * ncurves, s);
+    d_base.alloc(sizeof(int) * ncurves, s);
     d_nch.alloc(sizeof(int) * ncurves, s);
     keys.alloc(sizeof(int) * std::max<long>(n, 1), s);
-    rec_orig.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1));
-    hist.alloc(sizeof(int) * std::max<long>(hist_len, 1), s);
-    bsum.alloc(sizeof(int) * ceil_div(std::max<long>(hist_len, 1), kScanTile), s);
+    recA.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1), s);
+    permA.alloc(sizeof(int) * std::max<long>(n, 1), s);
+    hist.alloc(sizeof(int) * hist_len, s);
+    bsum.alloc(sizeof(int) * ceil_div(hist_len, kScanTile), s);
     if (!chunks.empty())
         CUNA_HIP(hipMemcpyAsync(d_chunks.ptr, chunks.data(), sizeof(int4) * chunks.size(),
                                 hipMemcpyHostToDevice, s));
-    CUNA_HIP(hipMemcpyAsync(d_mbase.ptr, mbase.data(), sizeof(long) * ncurves,
-                            hipMemcpyHostToDevice, s));
+    CUNA_HIP(hipMemcpyAsync(d_base.ptr, base.data(), sizeof(int) * ncurves, hipMemcpyHostToDevice,
+                            s));
     CUNA_HIP(hipMemcpyAsync(d_nch.ptr, nch.data(), sizeof(int) * ncurves, hipMemcpyHostToDevice,
                             s));
-    A.G = G;
-    A.tlog = tlog;
-    A.ntiles = ntiles;
-    A.Kc = Kc;
-    A.olo = olo;
-    A.ohi = olo + ntaps - 1;
-    A.periodic = periodic;
-    SortWork<T> W;
+    SortWork W;
     W.chunks = d_chunks.as<int4>();
-    W.mbase = d_mbase.as<long>();
+    W.chunk_base = d_base.as<int>();
     W.nch = d_nch.as<int>();
     W.coff = coff.as<int>();
     W.nchunks = (int)chunks.size();
     W.ncurves = ncurves;
-    W.hist_len = hist_len;
-    W.keys = keys.as<int>();
-    W.rec_tmp = rec_orig.as<PointRec<T>>();
+    W.total_chunks = total_chunks;
     W.hist = hist.as<int>();
     W.bsum = bsum.as<int>();
-    if (n > 0 || A.E3) {
-        if (chunks.empty()) {
-            // no points: buckets all empty
-            long nbk = (long)ncurves * (Kc + 1);
-            hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbk, 256)), dim3(256), 0, s,
-                               bucket.as<int>(), nbk, 0);
-            CUNA_LAUNCH_CHECK();
-        } else {
-            launch_sort<T>(A, W, rec.as<PointRec<T>>(), perm.as<int>(), bucket.as<int>(), s);
-        }
+    // phase reduction / compat decisions, then an LSD sort: by the cell within
+    // the tile, then (stably) by tile and edge class -> sorted by anchor
+    A.n = n;
+    A.G = G;
+    A.tlog = tlog;
+    A.rec = rec_orig.as<PointRec<T>>();
+    A.key = keys.as<int>();
+    launch_point_setup<T>(A, s);
+    launch_sort_pass<T>(W, K1, keys.as<int>(), rec_orig.as<PointRec<T>>(), nullptr,
+                        recA.as<PointRec<T>>(), permA.as<int>(), s);
+    launch_tile_keys<T>(recA.as<PointRec<T>>(), n, tlog, olo, olo + ntaps - 1, G, keys.as<int>(),
+                        s);
+    launch_sort_pass<T>(W, Kc, keys.as<int>(), recA.as<PointRec<T>>(), permA.as<int>(),
+                        rec.as<PointRec<T>>(), perm.as<int>(), s);
+    if (chunks.empty()) {
+        long nbk = (long)ncurves * (Kc + 1);
+        hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbk, 256)), dim3(256), 0, s,
+                           bucket.as<int>(), nbk, 0);
+        CUNA_LAUNCH_CHECK();
+    } else {
+        launch_buckets(W, Kc, bucket.as<int>(), s);
     }
     // work-queue order of the curves for the fused batched kernels
     std::vector<int> ord(ncurves);
@@ -183,7 +187,7 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
                             hipMemcpyHostToDevice, s));
     // the host tables stay alive with the plan: the copies above are async
     chunks_h = std::move(chunks);
-    mbase_h = std::move(mbase);
+    base_h = std::move(base);
     nch_h = std::move(nch);
 }
 
@@ -204,6 +208,7 @@ SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
     A.nrep = 1;
     A.periodic = P.periodic ? 1 : 0;
     A.wrap1 = (P.periodic && P.ntiles == 1) ? 1 : 0;
+    A.cap = sizeof(T) == 4 ? 2048 : 1024;
     A.bi = T(1) / beta;
     A.wscale = wscale;
     for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
@@ -282,6 +287,8 @@ template <typename T>
 void Plan<T>::prepare_bootstrap(int nbs, bool window) {
     if (bootstrap_fused_ok(window)) {
         counter.ensure(sizeof(int));
+        ymax.ensure(sizeof(T) * (size_t)ncurves());
+        cover.ensure(sizeof(int) * 2 * (size_t)ncurves());
         return;
     }
     counts.ensure(sizeof(int) * (size_t)std::max<long>(n(), 1) * nbs);
@@ -298,10 +305,23 @@ void Plan<T>::bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx
     const int nc = ncurves();
     prepare_bootstrap(nbs, out_w != nullptr);
     if (bootstrap_fused_ok(out_w != nullptr)) {
+        int *cov = cover.as<int>();
+        if (!cover_ready) {
+            for (int w = 0; w < (has_pts2 ? 2 : 1); ++w) {
+                const TiledPoints<T> &P = w == 0 ? pts : pts2;
+                launch_curve_cover<T>(P.rec_orig.template as<PointRec<T>>(),
+                                      P.coff.template as<int>(), nc, G, ntaps, cov + w * nc, s);
+            }
+            cover_ready = true;
+        }
+        if (y) launch_curve_absmax<T>(y, pts.coff.template as<int>(), nc, ymax.as<T>(), s);
         BootArgs<T> B{};
         B.rec_x = pts.rec_orig.template as<PointRec<T>>();
         B.rec_2x = has_pts2 ? pts2.rec_orig.template as<PointRec<T>>() : nullptr;
         B.y = y;
+        B.ymax = ymax.as<T>();
+        B.cover_x = cov;
+        B.cover_2x = cov + nc;
         B.coff = pts.coff.template as<int>();
         B.order = pts.order.template as<int>();
         B.counter = counter.as<int>();

@@ -291,27 +291,52 @@ void fft_complex(const FftShape &sh, cplx<T> *data, cplx<T> *work, int sign, hip
 
 // ----------------------------------------------------------------- sort --
 template <typename T>
-void launch_sort(SetupArgs<T> A, const SortWork<T> &W, PointRec<T> *rec, int *perm, int *bucket,
-                 hipStream_t s) {
-    A.chunks = W.chunks;
-    A.mbase = W.mbase;
-    A.nch = W.nch;
-    A.nchunks = W.nchunks;
-    A.keys = W.keys;
-    A.rec_tmp = W.rec_tmp;
-    A.hist = W.hist;
+void launch_point_setup(const SetupArgs<T> &A, hipStream_t s) {
+    const long n = std::max(A.n, (long)A.R_f);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_point_setup<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, A);
+    CUNA_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_tile_keys(const PointRec<T> *rec, long n, int tlog, int olo, int ohi, int G, int *key,
+                      hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_tile_keys<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, rec, n, tlog, olo,
+                       ohi, G, key);
+    CUNA_LAUNCH_CHECK();
+}
+
+static SortArgs sort_args(const SortWork &W, int K, const int *keys) {
+    SortArgs S;
+    S.chunks = W.chunks;
+    S.chunk_base = W.chunk_base;
+    S.nch = W.nch;
+    S.nchunks = W.nchunks;
+    S.K = K;
+    S.kbits = ilog2(K);
+    S.keys = keys;
+    S.hist = W.hist;
+    return S;
+}
+
+template <typename T>
+void launch_sort_pass(const SortWork &W, int K, const int *keys, const PointRec<T> *rec_in,
+                      const int *perm_in, PointRec<T> *rec_out, int *perm_out, hipStream_t s) {
+    if (W.nchunks == 0) return;
+    const SortArgs S = sort_args(W, K, keys);
     // waves per block limited by the per-wave LDS counters
-    size_t per_wave = sizeof(int) * (size_t)A.Kc;
-    require(per_wave <= kMaxLds, "too many grid tiles for one curve", CUNA_ERROR_UNSUPPORTED);
+    size_t per_wave = sizeof(int) * (size_t)K;
+    require(per_wave <= kMaxLds, "too many sort keys", CUNA_ERROR_UNSUPPORTED);
     int wpb = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per_wave));
     size_t lds = per_wave * wpb;
-    int nblk = (int)ceil_div(std::max(W.nchunks, 1), wpb);
-    set_lds(k_setup_hist<T>, lds);
+    int nblk = (int)ceil_div(W.nchunks, wpb);
+    set_lds(k_hist, lds);
     set_lds(k_scatter<T>, lds);
-    hipLaunchKernelGGL(k_setup_hist<T>, dim3(nblk), dim3(64 * wpb), lds, s, A);
+    hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(64 * wpb), lds, s, S);
     CUNA_LAUNCH_CHECK();
     // exclusive scan of the key-major count matrix
-    const long n = W.hist_len;
+    const long n = (long)K * W.total_chunks;
     const long nb = ceil_div(std::max(n, 1L), kScanTile);
     size_t slds = sizeof(int) * kScanNT;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanNT), slds, s, W.hist, n, W.bsum);
@@ -320,27 +345,24 @@ void launch_sort(SetupArgs<T> A, const SortWork<T> &W, PointRec<T> *rec, int *pe
     CUNA_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanNT), slds, s, W.hist, n, W.bsum, W.hist);
     CUNA_LAUNCH_CHECK();
-    ScatterArgs S;
-    S.chunks = W.chunks;
-    S.mbase = W.mbase;
-    S.nch = W.nch;
-    S.nchunks = W.nchunks;
-    S.Kc = A.Kc;
-    S.kbits = ilog2(A.Kc);
-    S.keys = W.keys;
-    S.offsets = W.hist;
-    hipLaunchKernelGGL(k_scatter<T>, dim3(nblk), dim3(64 * wpb), lds, s, S, W.rec_tmp, rec, perm);
+    hipLaunchKernelGGL(k_scatter<T>, dim3(nblk), dim3(64 * wpb), lds, s, S, rec_in, perm_in,
+                       rec_out, perm_out);
     CUNA_LAUNCH_CHECK();
-    long nbk = (long)W.ncurves * (A.Kc + 1);
-    hipLaunchKernelGGL(k_buckets, dim3(ceil_div(nbk, 256)), dim3(256), 0, s, W.hist, W.mbase, W.nch,
-                       W.coff, W.ncurves, A.Kc, bucket);
+}
+
+void launch_buckets(const SortWork &W, int K, int *bucket, hipStream_t s) {
+    const SortArgs S = sort_args(W, K, nullptr);
+    long nbk = (long)W.ncurves * (K + 1);
+    hipLaunchKernelGGL(k_buckets, dim3(ceil_div(nbk, 256)), dim3(256), 0, s, S, W.coff, W.ncurves,
+                       bucket);
     CUNA_LAUNCH_CHECK();
 }
 
 // --------------------------------------------------------------- spread --
 template <typename T, int NTAPS, int VM>
 static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s) {
-    size_t lds = sizeof(T) * 4 * ((size_t)1 << A.tlog);
+    const size_t lds = spread_lds_bytes<T>(A.tlog, NTAPS, A.cap);
+    require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
     auto k = k_spread<T, NTAPS, VM>;
     set_lds(k, lds);
     hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
@@ -405,7 +427,8 @@ void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s)
 
 // ------------------------------------------------------ fused bootstrap --
 // grids up to 2^15 cells (float) / 2^14 (double): 128 KB of LDS, so one
-// persistent workgroup of 1024 threads per CU
+// persistent workgroup per CU, at most 512 threads (the in-place FFT then
+// holds two radix-16 butterflies per thread without spilling)
 template <typename T>
 bool boot_fused_supported(int lgN, int ntaps) {
     return lgN >= 6 && lgN <= max_small_lg<T>() && (ntaps == 12 || ntaps == 16 || ntaps == 20 ||
@@ -414,7 +437,7 @@ bool boot_fused_supported(int lgN, int ntaps) {
 
 template <typename T, int LG, int NTAPS>
 static void boot_fused_one(BootArgs<T> A, hipStream_t s) {
-    constexpr int NT = fft_threads(LG, 1) < 256 ? 256 : fft_threads(LG, 1);
+    constexpr int NT = fft_threads(LG, 1) < 256 ? 256 : (fft_threads(LG, 1) > 512 ? 512 : fft_threads(LG, 1));
     auto &tc = TwiddleCache<T>::get();
     A.tw = tc.fine(LG);
     A.twG = tc.two_level(LG + 1);
@@ -431,6 +454,25 @@ static void boot_fused_one(BootArgs<T> A, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, A.counter);
     CUNA_LAUNCH_CHECK();
     hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, A);
+    CUNA_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_curve_cover(const PointRec<T> *rec, const int *coff, int ncurves, int G, int ntaps,
+                        int *cover, hipStream_t s) {
+    require(is_pow2(G) && G <= (1 << 15), "cover histogram needs a power-of-two grid <= 2^15");
+    if (ncurves <= 0) return;
+    const size_t lds = sizeof(int) * (size_t)G;
+    set_lds(k_curve_cover<T>, lds);
+    hipLaunchKernelGGL(k_curve_cover<T>, dim3(ncurves), dim3(1024), lds, s, rec, coff, G, ntaps,
+                       cover);
+    CUNA_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_curve_absmax(const T *y, const int *coff, int ncurves, T *out, hipStream_t s) {
+    if (ncurves <= 0) return;
+    hipLaunchKernelGGL(k_curve_absmax<T>, dim3(ncurves), dim3(256), 0, s, y, coff, out);
     CUNA_LAUNCH_CHECK();
 }
 
@@ -459,11 +501,17 @@ template void fft_real_forward<dTyp>(const FftShape &, dTyp *, const SinkCompat<
 template void fft_c2r_inverse<dTyp>(const FftShape &, const LoadHermitian<dTyp> &, cplx<dTyp> *,
                                     dTyp *, hipStream_t);
 template void fft_complex<dTyp>(const FftShape &, cplx<dTyp> *, cplx<dTyp> *, int, hipStream_t);
-template void launch_sort<dTyp>(SetupArgs<dTyp>, const SortWork<dTyp> &, PointRec<dTyp> *, int *,
-                                int *, hipStream_t);
+template void launch_point_setup<dTyp>(const SetupArgs<dTyp> &, hipStream_t);
+template void launch_sort_pass<dTyp>(const SortWork &, int, const int *, const PointRec<dTyp> *,
+                                     const int *, PointRec<dTyp> *, int *, hipStream_t);
+template void launch_tile_keys<dTyp>(const PointRec<dTyp> *, long, int, int, int, int, int *,
+                                     hipStream_t);
 template void launch_spread<dTyp>(const SpreadArgs<dTyp> &, int, int, int, hipStream_t);
 template void launch_gather<dTyp>(const GatherArgs<dTyp> &, int, int, hipStream_t);
 template bool boot_fused_supported<dTyp>(int, int);
 template void launch_boot_fused<dTyp>(BootArgs<dTyp>, int, int, hipStream_t);
+template void launch_curve_cover<dTyp>(const PointRec<dTyp> *, const int *, int, int, int, int *,
+                                       hipStream_t);
+template void launch_curve_absmax<dTyp>(const dTyp *, const int *, int, dTyp *, hipStream_t);
 
 }  // namespace cuna

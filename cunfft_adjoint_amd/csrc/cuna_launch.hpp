@@ -47,22 +47,26 @@ template <typename T>
 void fft_complex(const FftShape &sh, cplx<T> *data, cplx<T> *work, int sign, hipStream_t s);
 
 // sort-by-tile plan building
-template <typename T>
 struct SortWork {
     const int4 *chunks;
-    const long *mbase;
+    const int *chunk_base;
     const int *nch;
     const int *coff;
-    int nchunks, ncurves;
-    long hist_len;
-    int *keys;
-    PointRec<T> *rec_tmp;
-    int *hist;
+    int nchunks, ncurves, total_chunks;
+    int *hist;  // >= K * total_chunks entries for the largest K used
     int *bsum;
 };
 template <typename T>
-void launch_sort(SetupArgs<T> A, const SortWork<T> &W, PointRec<T> *rec, int *perm, int *bucket,
-                 hipStream_t s);
+void launch_point_setup(const SetupArgs<T> &A, hipStream_t s);
+// one stable counting-sort pass over keys in [0, K)
+template <typename T>
+void launch_sort_pass(const SortWork &W, int K, const int *keys, const PointRec<T> *rec_in,
+                      const int *perm_in, PointRec<T> *rec_out, int *perm_out, hipStream_t s);
+template <typename T>
+void launch_tile_keys(const PointRec<T> *rec, long n, int tlog, int olo, int ohi, int G, int *key,
+                      hipStream_t s);
+// bucket offsets of the last sort pass (over K keys)
+void launch_buckets(const SortWork &W, int K, int *bucket, hipStream_t s);
 
 template <typename T>
 void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStream_t s);
@@ -73,6 +77,12 @@ void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s)
 // fused kernel
 template <typename T>
 bool boot_fused_supported(int lgN, int ntaps);
+// per-curve densest tap window (G <= 2^15) and max |y|
+template <typename T>
+void launch_curve_cover(const PointRec<T> *rec, const int *coff, int ncurves, int G, int ntaps,
+                        int *cover, hipStream_t s);
+template <typename T>
+void launch_curve_absmax(const T *y, const int *coff, int ncurves, T *out, hipStream_t s);
 template <typename T>
 void launch_boot_fused(BootArgs<T> A, int lgN, int ntaps, hipStream_t s);
 

@@ -45,7 +45,7 @@ struct TiledPoints {
     // host copies of the sort tables, declared first so that they are
     // destroyed after the device buffers (whose hipFree synchronises)
     std::vector<int4> chunks_h;
-    std::vector<long> mbase_h;
+    std::vector<int> base_h;
     std::vector<int> nch_h;
     std::vector<int> order_h;
     int ncurves = 0;
@@ -83,6 +83,9 @@ struct Plan {
     TiledPoints<T> pts, pts2;  // pts2: points 2x (window at 2f)
     bool has_pts2 = false;
     DevMem grid, work, counts, counter;
+    // fused bootstrap: per-curve max |y| and densest tap window of x and 2x
+    DevMem ymax, cover;
+    bool cover_ready = false;
     // optional profiling events, recorded on the execution stream before the
     // first kernel of adjoint() and after each of its kernels
     hipEvent_t prof[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -12,6 +12,8 @@
 // writes the tile once with coalesced stores.  No global atomics.
 #pragma once
 
+#include <type_traits>
+
 #include "cuna_common.hpp"
 #include "cuna_fft.hpp"
 
@@ -56,15 +58,10 @@ struct SetupArgs {
     T b, e1s;          // compat b and 1/sqrt(pi b)
     T *E1, *E2, *E3;   // compat filter_properties arrays (nullable)
     T *eps_orig;       // compat: eps by original index (nullable)
-    int tlog, ntiles, Kc, olo, ohi;
-    bool periodic;
-    const int4 *chunks;  // {curve, start, len, column}
-    const long *mbase;   // matrix base per curve
-    const int *nch;      // chunks per curve
-    int nchunks;
-    int *keys;
-    PointRec<T> *rec_tmp;
-    int *hist;
+    long n;
+    int tlog;          // tile = 2^tlog cells
+    PointRec<T> *rec;  // out: (anchor, e) by original index
+    int *key;          // out: anchor & (tile - 1), the first sort digit
 };
 
 // compat decisions in the build's precision, one rounding per operation
@@ -118,6 +115,25 @@ __device__ __forceinline__ void setup_point(const SetupArgs<T> &A, long i, int &
     }
 }
 
+template <typename T>
+__global__ void k_point_setup(SetupArgs<T> A) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (A.E3 && i < A.R_f) {
+        // src/gaussian_filter.cu:222-226: E3[m] = exp(-m^2/b)
+        const int m = (int)i;
+        A.E3[m] = (T)exp(-(T)(m * m) / A.b);
+    }
+    if (i >= A.n) return;
+    int a;
+    T e;
+    setup_point(A, i, a, e);
+    PointRec<T> pr;
+    pr.a = a;
+    pr.e = e;
+    A.rec[i] = pr;
+    A.key[i] = a & ((1 << A.tlog) - 1);
+}
+
 __device__ __forceinline__ int tile_key(int a, int tlog, int olo, int ohi, int G) {
     int tile = a >> tlog;
     int ts = tile << tlog;
@@ -126,70 +142,61 @@ __device__ __forceinline__ int tile_key(int a, int tlog, int olo, int ohi, int G
     return 3 * tile + cls;
 }
 
-// one wave per chunk, blockDim/64 chunks per block; LDS counters per wave
+// second sort digit: 3*tile + class (class 0: taps reach the previous tile,
+// 2: the next one, 1: interior)
 template <typename T>
-__global__ __launch_bounds__(256) void k_setup_hist(SetupArgs<T> A) {
+__global__ void k_tile_keys(const PointRec<T> *rec, long n, int tlog, int olo, int ohi, int G,
+                            int *key) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) key[s] = tile_key(rec[s].a, tlog, olo, ohi, G);
+}
+
+// A stable counting sort pass over K keys.  Points are cut into chunks that
+// never cross a curve; the per-chunk counts form a key-major matrix
+// hist[curve base + key * nch + chunk], whose exclusive scan gives every
+// chunk's first position for every key.  One wave handles one chunk.
+struct SortArgs {
+    const int4 *chunks;     // {curve, start, len, column}
+    const int *chunk_base;  // chunks before each curve
+    const int *nch;         // chunks per curve
+    int nchunks, K, kbits;
+    const int *keys;
+    int *hist;
+};
+
+__device__ __forceinline__ long hist_index(const SortArgs &S, int4 cd, int k) {
+    return (long)S.K * S.chunk_base[cd.x] + (long)k * S.nch[cd.x] + cd.w;
+}
+
+static __global__ void k_hist(SortArgs S) {
     int *cnt_all = reinterpret_cast<int *>(cuna_smem);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ch = blockIdx.x * (blockDim.x >> 6) + wave;
-    int *cnt = cnt_all + wave * A.Kc;
-    for (int k = lane; k < A.Kc; k += 64) cnt[k] = 0;
+    int *cnt = cnt_all + wave * S.K;
+    for (int k = lane; k < S.K; k += 64) cnt[k] = 0;
     __syncthreads();
-    int4 cd = make_int4(0, 0, 0, 0);
-    if (ch < A.nchunks) {
-        cd = A.chunks[ch];
-        for (int r0 = 0; r0 < cd.z; r0 += 64) {
-            int q = r0 + lane;
-            if (q < cd.z) {
-                long i = (long)cd.y + q;
-                int a;
-                T e;
-                setup_point(A, i, a, e);
-                int key = tile_key(a, A.tlog, A.olo, A.ohi, A.G);
-                A.keys[i] = key;
-                PointRec<T> pr;
-                pr.a = a;
-                pr.e = e;
-                A.rec_tmp[i] = pr;
-                atomicAdd(&cnt[key], 1);
-            }
-        }
-    }
-    if (ch == 0 && A.E3 && threadIdx.x < A.R_f) {
-        // src/gaussian_filter.cu:222-226: E3[m] = exp(-m^2/b)
-        int m = threadIdx.x;
-        A.E3[m] = (T)exp(-(T)(m * m) / A.b);
-    }
-    __syncthreads();
-    if (ch < A.nchunks) {
-        const long base = A.mbase[cd.x];
-        const int nc = A.nch[cd.x];
-        for (int k = lane; k < A.Kc; k += 64) A.hist[base + (long)k * nc + cd.w] = cnt[k];
-    }
-}
-
-struct ScatterArgs {
-    const int4 *chunks;
-    const long *mbase;
-    const int *nch;
-    int nchunks, Kc, kbits;
-    const int *keys;
-    const int *offsets;  // scanned matrix
-};
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_scatter(ScatterArgs S, const PointRec<T> *rec_tmp,
-                                                 PointRec<T> *rec, int *perm) {
-    int *cur_all = reinterpret_cast<int *>(cuna_smem);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ch = blockIdx.x * (blockDim.x >> 6) + wave;
-    int *cur = cur_all + wave * S.Kc;
     int4 cd = make_int4(0, 0, 0, 0);
     if (ch < S.nchunks) {
         cd = S.chunks[ch];
-        const long base = S.mbase[cd.x];
-        const int nc = S.nch[cd.x];
-        for (int k = lane; k < S.Kc; k += 64) cur[k] = S.offsets[base + (long)k * nc + cd.w];
+        for (int q = lane; q < cd.z; q += 64) atomicAdd(&cnt[S.keys[(long)cd.y + q]], 1);
+    }
+    __syncthreads();
+    if (ch < S.nchunks)
+        for (int k = lane; k < S.K; k += 64) S.hist[hist_index(S, cd, k)] = cnt[k];
+}
+
+// scatter with ranks that keep the input order among equal keys
+template <typename T>
+__global__ void k_scatter(SortArgs S, const PointRec<T> *rec_in, const int *perm_in,
+                          PointRec<T> *rec_out, int *perm_out) {
+    int *cur_all = reinterpret_cast<int *>(cuna_smem);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ch = blockIdx.x * (blockDim.x >> 6) + wave;
+    int *cur = cur_all + wave * S.K;
+    int4 cd = make_int4(0, 0, 0, 0);
+    if (ch < S.nchunks) {
+        cd = S.chunks[ch];
+        for (int k = lane; k < S.K; k += 64) cur[k] = S.hist[hist_index(S, cd, k)];
     }
     __syncthreads();
     if (ch < S.nchunks) {
@@ -211,8 +218,8 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs S, const PointRec<T
             if (act) pos = cur[key] + rank;
             __builtin_amdgcn_wave_barrier();
             if (act) {
-                rec[pos] = rec_tmp[i];
-                perm[pos] = (int)i;
+                rec_out[pos] = rec_in[i];
+                perm_out[pos] = perm_in ? perm_in[i] : (int)i;
                 if (rank == gcount - 1) cur[key] = pos + 1;
             }
             __builtin_amdgcn_wave_barrier();
@@ -220,18 +227,17 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs S, const PointRec<T
     }
 }
 
-// bucket[c*(Kc+1) + k] = first sorted index of key k of curve c
-static __global__ void k_buckets(const int *offsets, const long *mbase, const int *nch, const int *coff,
-                          int ncurves, int Kc, int *bucket) {
+// bucket[c*(K+1) + k] = first sorted index of key k of curve c
+static __global__ void k_buckets(SortArgs S, const int *coff, int ncurves, int *bucket) {
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long total = (long)ncurves * (Kc + 1);
+    long total = (long)ncurves * (S.K + 1);
     if (idx >= total) return;
-    int c = (int)(idx / (Kc + 1)), k = (int)(idx % (Kc + 1));
+    int c = (int)(idx / (S.K + 1)), k = (int)(idx % (S.K + 1));
     int v;
-    if (k == Kc || nch[c] == 0)
-        v = k == Kc ? coff[c + 1] : coff[c];
+    if (k == S.K || S.nch[c] == 0)
+        v = k == S.K ? coff[c + 1] : coff[c];
     else
-        v = offsets[mbase[c] + (long)k * nch[c]];
+        v = S.hist[(long)S.K * S.chunk_base[c] + (long)k * S.nch[c]];
     bucket[idx] = v;
 }
 
@@ -305,7 +311,7 @@ enum ValueMode { VAL_PLAIN = 0, VAL_COMPAT_BOOT = 1, VAL_COUNTS = 2 };
 
 template <typename T>
 struct SpreadArgs {
-    const PointRec<T> *rec;
+    const PointRec<T> *rec;  // points sorted by anchor within each curve
     const int *perm;
     const int *bucket;
     const int *coff;       // curve point offsets (ncurves + 1)
@@ -316,6 +322,7 @@ struct SpreadArgs {
     int G, tlog, ntiles, Kc, olo, ntaps, nrep;
     int periodic;
     int wrap1;  // periodic and a single tile covers the grid
+    int cap;    // points staged in LDS per round
     T bi;       // 1/beta
     T wscale;   // weight prefactor
     T e3[kMaxTaps];  // exp(-o^2/beta), o = olo + q
@@ -375,78 +382,55 @@ __device__ __forceinline__ void point_value(const SpreadArgs<T> &A, int p, int c
     }
 }
 
-template <typename T, int NTAPS>
-__device__ __forceinline__ void deposit(const SpreadArgs<T> &A, T *acc, int TT, int a, T e, T v,
-                                        int off) {
-    constexpr int Q0 = (NTAPS - 1) / 2;
-    T w[NTAPS];
-    tap_weights<T, NTAPS>(v * A.wscale, e, A.bi, A.e3, w);
-    const int rel = a - off - Q0;  // cell of tap q = 0
-    if (rel >= 0 && rel + NTAPS <= TT) {
-#pragma unroll
-        for (int q = 0; q < NTAPS; ++q) atomicAdd(&acc[rel + q], w[q]);
-    } else if (A.wrap1) {
-        // single periodic tile (TT = G, a power of two): taps wrap around
-#pragma unroll
-        for (int q = 0; q < NTAPS; ++q) atomicAdd(&acc[(rel + q) & (TT - 1)], w[q]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < NTAPS; ++q) {
-            const int cidx = rel + q;
-            if (cidx >= 0 && cidx < TT) atomicAdd(&acc[cidx], w[q]);
-        }
-    }
+template <typename T>
+struct StagedPoint {
+    int a;  // anchor relative to the tile's first cell
+    T e, v;
+};
+
+// LDS bytes of one spread workgroup
+template <typename T>
+__host__ __device__ inline size_t spread_lds_bytes(int tlog, int ntaps, int cap) {
+    size_t b = sizeof(T) * ((size_t)1 << tlog) + sizeof(int) * (((size_t)1 << tlog) + 2 * ntaps + 1);
+    b = (b + 15) & ~(size_t)15;
+    return b + sizeof(StagedPoint<T>) * (size_t)cap;
 }
 
-// Points [begin, end) of the sorted order, seen from the tile whose first
-// cell is `off`.  Each thread takes U points per round and issues all their
-// loads before using any, so that the dependent gathers overlap.
-template <typename T, int NTAPS, int VM>
-__device__ __forceinline__ void spread_range(const SpreadArgs<T> &A, T *acc, int TT, int begin,
-                                             int end, int off, int c, int rep) {
-    constexpr int NT = 256, U = 4;
-    for (int s0 = begin + (int)threadIdx.x; s0 < end; s0 += NT * U) {
-        PointRec<T> pr[U];
-        int p[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int s = s0 + u * NT;
-            if (s < end) {
-                pr[u] = A.rec[s];
-                p[u] = A.perm[s];
-            }
-        }
-        T v[U], e[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            e[u] = pr[u].e;
-            if (s0 + u * NT < end) point_value<T, VM>(A, p[u], c, rep, v[u], e[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (s0 + u * NT < end && (VM != VAL_COUNTS || v[u] != T(0)))
-                deposit<T, NTAPS>(A, acc, TT, pr[u].a, e[u], v[u], off);
-    }
-}
-
-// grid: blockIdx.x = tile, blockIdx.y = slab (curve * nrep + replicate)
+// grid: blockIdx.x = tile, blockIdx.y = slab (curve * nrep + replicate).
+// Output-driven: every cell of the tile is owned by one thread, which sums
+// v_i exp(-(o_i - e_i)^2/beta) over the points i whose taps reach it (o_i =
+// cell - anchor_i).  Points are sorted by anchor, so those are a contiguous
+// window of the staged list; start[L] (first point with anchor >= L) is built
+// in one pass over the points.  Deterministic: each cell sums in a fixed order.
 template <typename T, int NTAPS, int VM>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
-    constexpr int NT = 256, NW = NT / 64;
+    constexpr int NT = 256, Q0 = (NTAPS - 1) / 2, QH = NTAPS - 1 - Q0;
+    constexpr int LMIN = -NTAPS;
     const int TT = 1 << A.tlog;
-    T *acc_all = reinterpret_cast<T *>(cuna_smem);
+    const int LMAX = TT + NTAPS + 1;  // start[] covers cells LMIN .. LMAX-1
+    T *gacc = reinterpret_cast<T *>(cuna_smem);
+    int *start = reinterpret_cast<int *>(gacc + TT);
+    size_t off_pts = sizeof(T) * TT + sizeof(int) * (TT + 2 * NTAPS + 1);
+    off_pts = (off_pts + 15) & ~(size_t)15;
+    StagedPoint<T> *pts = reinterpret_cast<StagedPoint<T> *>(cuna_smem + off_pts);
+
     const int nt = A.ntiles;
     const int tile = xcd_remap(blockIdx.x, nt);
     const int slab = blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
     const int *bk = A.bucket + (long)c * (A.Kc + 1);
-    for (int i = threadIdx.x; i < NW * TT; i += NT) acc_all[i] = T(0);
-    __syncthreads();
-    T *acc = acc_all + (threadIdx.x >> 6) * TT;
     const int tstart = tile << A.tlog;
-    // own tile, all classes
-    spread_range<T, NTAPS, VM>(A, acc, TT, bk[3 * tile], bk[3 * tile + 3], tstart, c, rep);
-    if (nt > 1) {
+
+    // Ranges of sorted points, each with the offset that maps its anchors to
+    // this tile's cells, in increasing order of mapped anchor: the previous
+    // tile's right-edge class, this tile, the next tile's left-edge class.
+    // A tiny periodic grid (taps wrap more than once) instead uses 2S+1
+    // shifted copies of all its points.
+    const bool tiny = A.wrap1 && A.G < 2 * NTAPS;
+    const int S = tiny ? (NTAPS + A.G - 1) / A.G + 1 : 0;
+    int b0 = 0, n0 = 0, o0 = 0, b1 = bk[3 * tile], n1 = bk[3 * tile + 3] - b1, b2 = 0, n2 = 0,
+        o2 = 0;
+    if (!tiny) {
         int prev = tile - 1, next = tile + 1;
         bool hp = true, hn = true;
         if (prev < 0) {
@@ -457,22 +441,74 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
             next = 0;
             hn = A.periodic;
         }
-        if (hp)  // previous tile's right-edge class, seen from this tile
-            spread_range<T, NTAPS, VM>(A, acc, TT, bk[3 * prev + 2], bk[3 * prev + 3],
-                                       (prev << A.tlog) + TT, c, rep);
-        if (hn)  // next tile's left-edge class
-            spread_range<T, NTAPS, VM>(A, acc, TT, bk[3 * next], bk[3 * next + 1],
-                                       (next << A.tlog) - TT, c, rep);
+        if (hp) {
+            b0 = bk[3 * prev + 2];
+            n0 = bk[3 * prev + 3] - b0;
+            o0 = (prev << A.tlog) + TT;
+        }
+        if (hn) {
+            b2 = bk[3 * next];
+            n2 = bk[3 * next + 1] - b2;
+            o2 = (next << A.tlog) - TT;
+        }
+    }
+    const int total = tiny ? n1 * (2 * S + 1) : n0 + n1 + n2;
+
+    for (int g = threadIdx.x; g < TT; g += NT) gacc[g] = T(0);
+    for (int base = 0; base < total; base += A.cap) {
+        const int cnt = min(A.cap, total - base);
+        __syncthreads();
+        // stage this round's points: anchor in tile coordinates, e, value
+        for (int i = threadIdx.x; i < cnt; i += NT) {
+            const int j = base + i;
+            int s, ro;
+            if (tiny) {
+                s = b1 + j % n1;
+                ro = -(j / n1 - S) * A.G;
+            } else if (j < n0) {
+                s = b0 + j;
+                ro = o0;
+            } else if (j < n0 + n1) {
+                s = b1 + (j - n0);
+                ro = tstart;
+            } else {
+                s = b2 + (j - n0 - n1);
+                ro = o2;
+            }
+            const PointRec<T> pr = A.rec[s];
+            T v, e = pr.e;
+            point_value<T, VM>(A, A.perm[s], c, rep, v, e);
+            StagedPoint<T> sp;
+            sp.a = pr.a - ro;
+            sp.e = e;
+            sp.v = v;
+            pts[i] = sp;
+        }
+        for (int L = threadIdx.x; L < LMAX - LMIN; L += NT) start[L] = cnt;
+        __syncthreads();
+        // start[L - LMIN] = first staged point with anchor >= L
+        for (int i = threadIdx.x; i < cnt; i += NT) {
+            const int hi = min(pts[i].a, LMAX - 1);
+            const int lo = i == 0 ? LMIN : max(pts[i - 1].a + 1, LMIN);
+            for (int L = lo; L <= hi; ++L) start[L - LMIN] = i;
+        }
+        __syncthreads();
+        for (int g = threadIdx.x; g < TT; g += NT) {
+            const int lo = start[g - QH - LMIN];
+            const int hi = start[g + Q0 + 1 - LMIN];
+            T sum = T(0);
+            for (int i = lo; i < hi; ++i) {
+                const StagedPoint<T> sp = pts[i];
+                const T d = (T)(g - sp.a) - sp.e;
+                sum += sp.v * dexp(-d * d * A.bi);
+            }
+            gacc[g] += sum;
+        }
     }
     __syncthreads();
-    T *g = A.grid + (long)slab * A.grid_stride + tstart;
+    T *gout = A.grid + (long)slab * A.grid_stride + tstart;
     const int tlen = min(TT, A.G - tstart);
-    for (int i = threadIdx.x; i < tlen; i += NT) {
-        T s = acc_all[i];
-#pragma unroll
-        for (int w = 1; w < NW; ++w) s += acc_all[w * TT + i];
-        g[i] = s;
-    }
+    for (int g = threadIdx.x; g < tlen; g += NT) gout[g] = gacc[g] * A.wscale;
 }
 
 // ------------------------------------------- fused batched bootstrap --
@@ -482,11 +518,22 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
 // into a grid of G = 2N cells held entirely in LDS, transforms it in place
 // (real input as N complex) and writes the deconvolved spectrum:
 // transform 0 is y at the points x_d, transform 1 the unit-weight window at 2 x_d.
+//
+// LDS float atomics are slow on this hardware, so the grid accumulates in
+// fixed point with integer atomics (int32 in the FP32 library, int64 in FP64),
+// which are also order-independent.  The power-of-two scale comes from the
+// curve's max |y| and its densest tap window; an addition that overflows is
+// detected from the returned old value and the item is redone at a smaller
+// scale.  The integer grid occupies the same LDS words as the float grid and
+// is converted in place before the FFT.
 template <typename T>
 struct BootArgs {
     const PointRec<T> *rec_x;   // (anchor, frac) of x by original index
     const PointRec<T> *rec_2x;  // of 2x (window), nullable
     const T *y;                 // nullable: unit weights
+    const T *ymax;              // per curve max |y| (when y is set)
+    const int *cover_x;         // per curve: most points within one tap window
+    const int *cover_2x;
     const int *coff;
     const int *order;           // curves, longest first
     int *counter;               // work-queue head, zeroed before the launch
@@ -504,15 +551,34 @@ static __global__ void k_reset_counter(int *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
 }
 
+template <typename T>
+struct FixedOf;
+template <>
+struct FixedOf<float> {
+    using I = int;
+    static constexpr int headroom_bits = 30;
+    __device__ static I round(float v) { return __float2int_rn(v); }
+};
+template <>
+struct FixedOf<double> {
+    using I = long long;
+    static constexpr int headroom_bits = 61;
+    __device__ static I round(double v) { return __double2ll_rn(v); }
+};
+
 template <typename T, int LG, int NTAPS, int NT>
 __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
-    constexpr int N = 1 << LG, G = 2 * N, Q0 = (NTAPS - 1) / 2, U = 4;
+    using I = typename FixedOf<T>::I;
+    using U = std::make_unsigned_t<I>;
+    static_assert(sizeof(I) == sizeof(T), "fixed-point grid aliases the real grid");
+    constexpr int N = 1 << LG, G = 2 * N, Q0 = (NTAPS - 1) / 2;
+    constexpr int kMaxExp = sizeof(T) == 4 ? 100 : 1000;
     T *grid = reinterpret_cast<T *>(cuna_smem);
+    U *gu = reinterpret_cast<U *>(cuna_smem);
     cplx<T> *z = reinterpret_cast<cplx<T> *>(cuna_smem);
-    __shared__ int item_sh;
+    __shared__ int item_sh, overflow_sh;
     for (;;) {
         if (threadIdx.x == 0) item_sh = atomicAdd(A.counter, 1);
-        for (int i = threadIdx.x; i < G; i += NT) grid[i] = T(0);
         __syncthreads();
         const int item = item_sh;
         if (item >= A.total) break;  // uniform over the workgroup
@@ -523,29 +589,50 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
         const int c0 = A.coff[ci];
         const int nc = A.coff[ci + 1] - c0;
         const PointRec<T> *P = tr == 0 ? A.rec_x : A.rec_2x;
-        for (int p0 = threadIdx.x; p0 < nc; p0 += NT * U) {
-            PointRec<T> pr[U];
-            T v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int p = p0 + u * NT;
-                if (p < nc) {
+        const bool usey = tr == 0 && A.y;
+        const double vm = usey ? (double)A.ymax[ci] : 1.0;
+        int sexp = 0;
+        if (vm <= 1e300) {  // false for NaN and inf
+            // first attempt: 4x the densest window of the original points;
+            // second: nc points in one window, which cannot overflow
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                const int cnt = attempt == 0 ? 4 * max(tr == 0 ? A.cover_x[ci] : A.cover_2x[ci], 1)
+                                             : max(nc, 1);
+                const double bound = vm * (double)cnt * (double)A.wscale;
+                const int e = bound > 0 ? (int)ceil(log2(bound)) : -kMaxExp;
+                sexp = min(max(FixedOf<T>::headroom_bits - e, -kMaxExp), kMaxExp);
+                const T mul = (T)ldexp((double)A.wscale, sexp);
+                for (int i = threadIdx.x; i < G; i += NT) gu[i] = U(0);
+                if (threadIdx.x == 0) overflow_sh = 0;
+                __syncthreads();
+                bool ovf = false;
+                for (int p = threadIdx.x; p < nc; p += NT) {
                     const uint32_t t = (uint32_t)rep * (uint32_t)nc + (uint32_t)p;
                     const int d = c0 + (int)(xorshift32(wang_hash(A.seed + t)) % (uint32_t)nc);
-                    pr[u] = P[d];
-                    v[u] = (tr == 0 && A.y) ? A.y[d] : T(1);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (p0 + u * NT < nc) {
+                    const PointRec<T> pr = P[d];
+                    const T v = usey ? A.y[d] : T(1);
                     T w[NTAPS];
-                    tap_weights<T, NTAPS>(v[u] * A.wscale, pr[u].e, A.bi, A.e3, w);
-                    const int base = pr[u].a - Q0;
+                    tap_weights<T, NTAPS>(v * mul, pr.e, A.bi, A.e3, w);
+                    const int base = pr.a - Q0;
 #pragma unroll
-                    for (int q = 0; q < NTAPS; ++q) atomicAdd(&grid[(base + q) & (G - 1)], w[q]);
+                    for (int q = 0; q < NTAPS; ++q) {
+                        const I add = FixedOf<T>::round(w[q]);
+                        const I old = (I)atomicAdd(&gu[(base + q) & (G - 1)], (U)add);
+                        const I nw = (I)((U)old + (U)add);
+                        ovf |= ((old ^ nw) & (add ^ nw)) < 0;  // signed wrap
+                    }
                 }
+                if (ovf) overflow_sh = 1;
+                __syncthreads();
+                const bool redo = overflow_sh != 0;
+                __syncthreads();
+                if (!redo) break;
             }
+            // fixed point -> real, in place
+            const T inv = (T)ldexp(1.0, -sexp);
+            for (int i = threadIdx.x; i < G; i += NT) grid[i] = (T)(I)gu[i] * inv;
+        } else {
+            for (int i = threadIdx.x; i < G; i += NT) grid[i] = (T)NAN;
         }
         __syncthreads();
         lds_fft_inplace<T, 1, LG, NT>(z, A.tw);
@@ -555,6 +642,50 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
             emit_pair(sk, o, (long)r, (long)N, z[r], z[(N - r) & (N - 1)], A.twG(r));
         __syncthreads();
     }
+}
+
+// per-curve max |y|
+template <typename T>
+__global__ void k_curve_absmax(const T *y, const int *coff, T *out) {
+    const int c = blockIdx.x;
+    const int b = coff[c], e = coff[c + 1];
+    T m = T(0);
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) m = fmax(m, fabs(y[i]));
+    __shared__ T red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// per-curve densest window: the most points whose anchors fall in one
+// cyclic run of NTAPS cells of the G-cell grid.  One workgroup per curve,
+// anchor histogram in LDS (G ints).
+template <typename T>
+__global__ void k_curve_cover(const PointRec<T> *rec, const int *coff, int G, int ntaps,
+                              int *cover) {
+    int *h = reinterpret_cast<int *>(cuna_smem);
+    const int c = blockIdx.x;
+    for (int i = threadIdx.x; i < G; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int i = coff[c] + threadIdx.x; i < coff[c + 1]; i += blockDim.x)
+        atomicAdd(&h[rec[i].a & (G - 1)], 1);
+    __syncthreads();
+    int best = 0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        int sum = 0;
+        for (int k = 0; k < ntaps; ++k) sum += h[(g + k) & (G - 1)];
+        best = max(best, sum);
+    }
+    __shared__ int red;
+    if (threadIdx.x == 0) red = 0;
+    __syncthreads();
+    atomicMax(&red, best);
+    __syncthreads();
+    if (threadIdx.x == 0) cover[c] = red;
 }
 
 // ----------------------------------------------------------- gather --
