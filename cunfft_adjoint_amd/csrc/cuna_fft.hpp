@@ -128,9 +128,18 @@ constexpr int pass_log2(int lg, int p) {
     return p < n16 ? 4 : rem;
 }
 
+// ------------------------------------------------------------ LDS layout --
+// Element a of the LDS FFT buffer is stored at lds_sw(a): bits 0-3 are XORed
+// with bits 4-7 and bit 4 with bit 8.  The map is a bijection within every
+// aligned block of 32 elements, and it makes both the unit-stride reads and
+// the stride-16 writes of a radix-16 pass (8- and 16-byte elements, 32 or 64
+// banks) conflict-free; unswizzled, those writes hit 2 of the banks per
+// 64-byte group.
+__device__ __forceinline__ int lds_sw(int a) { return a ^ ((a >> 4) & 31); }
+
 // ----------------------------------------------------------- LDS Stockham --
 // W interleaved sequences of length L = 2^LG: element e of sequence w lives at
-// lds[e*W + w].  Pass 0 reads its inputs through `ld(e, w)`; every pass writes
+// lds[lds_sw(e*W + w)].  Pass 0 reads its inputs through `ld(e, w)`; every pass writes
 // LDS; the routine returns with natural-order results in LDS after a barrier.
 // tw[e] = exp(+2 pi i e / L) (conjugated for S = -1).
 template <typename T, int S, int LG, int W, int NT, int P, int LOGNS, class Loader,
@@ -147,7 +156,11 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
         constexpr int BPT = (NB + NT - 1) / NT;
         constexpr bool EXACT = (NB % NT) == 0;
         cplx<T> v[BPT][R];
-        const int tid = threadIdx.x;
+        // opaque thread id: in a persistent loop the compiler would otherwise
+        // hoist every pass's LDS addresses and twiddles out of the loop and
+        // keep them all live (spilling at 1024 threads)
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
 #pragma unroll
         for (int q = 0; q < BPT; ++q) {
             const int g = tid + q * NT;
@@ -159,7 +172,19 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
                     if constexpr (P == 0)
                         v[q][r] = ld(e, w);
                     else
-                        v[q][r] = lds[e * W + w];
+                        v[q][r] = lds[lds_sw(e * W + w)];
+                }
+                // twiddles are global loads: issue them before the barrier
+                // (with several butterflies per thread that would hold all
+                // their twiddles in registers at once; they follow it then)
+                if constexpr (P > 0 && BPT == 1) {
+                    const int k = j & ((1 << LOGNS) - 1);
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        cplx<T> t = tw[(k * r) << (LG - LOGNS - LR)];
+                        if constexpr (S < 0) t.y = -t.y;
+                        v[q][r] = v[q][r] * t;
+                    }
                 }
             }
         }
@@ -172,7 +197,7 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
             if (EXACT || g < NB) {
                 const int w = g % W, j = g / W;
                 const int k = j & ((1 << LOGNS) - 1);
-                if constexpr (P > 0) {
+                if constexpr (P > 0 && BPT > 1) {
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
                         cplx<T> t = tw[(k * r) << (LG - LOGNS - LR)];
@@ -183,7 +208,7 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
                 dft<T, R, S>(v[q]);
                 const int base = ((j >> LOGNS) << (LOGNS + LR)) + k;
 #pragma unroll
-                for (int r = 0; r < R; ++r) lds[(base + (r << LOGNS)) * W + w] = v[q][r];
+                for (int r = 0; r < R; ++r) lds[lds_sw((base + (r << LOGNS)) * W + w)] = v[q][r];
             }
         }
         __syncthreads();
@@ -197,7 +222,8 @@ __device__ __forceinline__ void lds_fft(cplx<T> *lds, const cplx<T> *__restrict_
     fft_pass<T, S, LG, W, NT, 0, 0, Loader>(lds, tw, ld);
 }
 
-// the data is already in `lds` (natural order); transformed in place
+// the data is already in `lds` in natural (unswizzled) order; transformed in
+// place, the result is in the swizzled layout
 template <typename T, int S, int LG, int NT>
 __device__ __forceinline__ void lds_fft_inplace(cplx<T> *lds, const cplx<T> *__restrict__ tw) {
     auto fetch = [&](int e, int) { return lds[e]; };
@@ -380,13 +406,13 @@ __global__ __launch_bounds__(NT) void k_fft_small(Loader ld, Sink sk, cplx<T> *p
     if constexpr (MODE == 0) {
         cplx<T> *o = sk.out + b * sk.ostride;
         for (long r = threadIdx.x; r <= N / 2; r += NT) {
-            cplx<T> Zr = lds[r];
-            cplx<T> Zn = lds[(N - r) & (N - 1)];
+            cplx<T> Zr = lds[lds_sw((int)r)];
+            cplx<T> Zn = lds[lds_sw((int)((N - r) & (N - 1)))];
             emit_pair(sk, o, r, N, Zr, Zn, twG(r));
         }
     } else {
         cplx<T> *o = plain_out + b * N;
-        for (long e = threadIdx.x; e < N; e += NT) o[e] = lds[e];
+        for (long e = threadIdx.x; e < N; e += NT) o[e] = lds[lds_sw((int)e)];
     }
 }
 
@@ -411,7 +437,7 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
         const int w = idx % W, r2 = idx / W;
         cplx<T> t = twN((long)(n10 + w) * r2);
         if constexpr (S < 0) t.y = -t.y;
-        o[n10 + w + (long)L1 * r2] = lds[r2 * W + w] * t;
+        o[n10 + w + (long)L1 * r2] = lds[lds_sw(r2 * W + w)] * t;
     }
 }
 
@@ -446,22 +472,22 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
         const int rho = i * H + h;
         if (rho != 0) {
             long r = rho + (long)L2 * r1;
-            cplx<T> Zr = lds[r1 * W + h];
-            cplx<T> Zn = lds[(L1 - 1 - r1) * W + (H + h)];
+            cplx<T> Zr = lds[lds_sw(r1 * W + h)];
+            cplx<T> Zn = lds[lds_sw((L1 - 1 - r1) * W + (H + h))];
             emit_pair(sk, o, r, N, Zr, Zn, twG(r));
         } else {
             // row 0 (slot h=0): element r1 pairs with (L1 - r1) mod L1, r1 <= L1/2
             if (r1 <= L1 / 2) {
                 long r = (long)L2 * r1;
-                cplx<T> Zr = lds[r1 * W + 0];
-                cplx<T> Zn = lds[((L1 - r1) & (L1 - 1)) * W + 0];
+                cplx<T> Zr = lds[lds_sw(r1 * W + 0)];
+                cplx<T> Zn = lds[lds_sw(((L1 - r1) & (L1 - 1)) * W + 0)];
                 emit_pair(sk, o, r, N, Zr, Zn, twG(r));
             }
             // row L2/2 (slot H): element r1 pairs with L1-1-r1, r1 < L1/2
             if (r1 < L1 / 2) {
                 long r = L2 / 2 + (long)L2 * r1;
-                cplx<T> Zr = lds[r1 * W + H];
-                cplx<T> Zn = lds[(L1 - 1 - r1) * W + H];
+                cplx<T> Zr = lds[lds_sw(r1 * W + H)];
+                cplx<T> Zn = lds[lds_sw((L1 - 1 - r1) * W + H)];
                 emit_pair(sk, o, r, N, Zr, Zn, twG(r));
             }
         }
@@ -485,7 +511,7 @@ __global__ __launch_bounds__(NT) void k_fft_row_plain(const cplx<T> *src, long s
     cplx<T> *o = dst + b * dst_stride;
     for (int idx = threadIdx.x; idx < H * L1; idx += NT) {
         const int h = idx % H, r1 = idx / H;
-        o[(i * H + h) + (long)L2 * r1] = lds[r1 * H + h];
+        o[(i * H + h) + (long)L2 * r1] = lds[lds_sw(r1 * H + h)];
     }
 }
 

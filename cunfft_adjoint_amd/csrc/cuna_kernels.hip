@@ -427,8 +427,7 @@ void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s)
 
 // ------------------------------------------------------ fused bootstrap --
 // grids up to 2^15 cells (float) / 2^14 (double): 128 KB of LDS, so one
-// persistent workgroup per CU, at most 512 threads (the in-place FFT then
-// holds two radix-16 butterflies per thread without spilling)
+// persistent workgroup per CU
 template <typename T>
 bool boot_fused_supported(int lgN, int ntaps) {
     return lgN >= 6 && lgN <= max_small_lg<T>() && (ntaps == 12 || ntaps == 16 || ntaps == 20 ||
@@ -437,7 +436,7 @@ bool boot_fused_supported(int lgN, int ntaps) {
 
 template <typename T, int LG, int NTAPS>
 static void boot_fused_one(BootArgs<T> A, hipStream_t s) {
-    constexpr int NT = fft_threads(LG, 1) < 256 ? 256 : (fft_threads(LG, 1) > 512 ? 512 : fft_threads(LG, 1));
+    constexpr int NT = fft_threads(LG, 1) < 256 ? 256 : fft_threads(LG, 1);
     auto &tc = TwiddleCache<T>::get();
     A.tw = tc.fine(LG);
     A.twG = tc.two_level(LG + 1);

@@ -639,7 +639,8 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
         SinkAccurate<T> sk = A.sink;
         cplx<T> *o = (tr == 0 ? A.out_y : A.out_w) + ((long)ci * A.nbs + rep) * sk.M;
         for (int r = threadIdx.x; r <= N / 2; r += NT)
-            emit_pair(sk, o, (long)r, (long)N, z[r], z[(N - r) & (N - 1)], A.twG(r));
+            emit_pair(sk, o, (long)r, (long)N, z[lds_sw(r)], z[lds_sw((N - r) & (N - 1))],
+                      A.twG(r));
         __syncthreads();
     }
 }
