@@ -178,28 +178,42 @@ def bench_c2(args, torch, cu):
     stream.synchronize()
     ms_per_cycle = ev.ms(outer[0], outer[1]) / 1e7
     hold_ms = 1.5 * host_step * 1e3 * (K + 8) + 20.0
+
+    def held(fn):
+        """Enqueue K steps behind a spin kernel; returns (ms of the K steps,
+        host enqueue ms)."""
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            torch.cuda._sleep(int(hold_ms / ms_per_cycle))
+        ev.record(outer[0], sp)
+        th = time.perf_counter()
+        for k in range(K):
+            fn(k)
+        enq = (time.perf_counter() - th) * 1e3
+        ev.record(outer[1], sp)
+        stream.synchronize()
+        torch.cuda.synchronize()
+        return ev.ms(outer[0], outer[1]), enq
+
+    # the timed region: K transforms back to back, nothing else on the stream
+    total_ms, enqueue_ms = held(lambda k: plan.adjoint(yd, out, stream=stream))
+    ms_step = total_ms / K
+    host_bound = enqueue_ms > hold_ms
+    # a second pass with events around each kernel (per-kernel durations for
+    # the roofline; the events add small gaps, so this pass is not `value`)
     stage = ev.create(4 * K)
-    torch.cuda.synchronize()
-    with torch.cuda.stream(stream):
-        torch.cuda._sleep(int(hold_ms / ms_per_cycle))
-    ev.record(outer[0], sp)
-    th = time.perf_counter()
-    for k in range(K):
+
+    def staged(k):
         arr = (ctypes.c_void_p * 4)(*stage[4 * k:4 * k + 4])
         L.cuna_nfft_set_profile_events(plan.h, arr, 4)
         plan.adjoint(yd, out, stream=stream)
+
+    held(staged)
     L.cuna_nfft_set_profile_events(plan.h, None, 0)
-    enqueue_ms = (time.perf_counter() - th) * 1e3
-    ev.record(outer[1], sp)
-    stream.synchronize()
-    torch.cuda.synchronize()
-    total_ms = ev.ms(outer[0], outer[1])
     spread = np.mean([ev.ms(stage[4 * k], stage[4 * k + 1]) for k in range(K)])
     colp = np.mean([ev.ms(stage[4 * k + 1], stage[4 * k + 2]) for k in range(K)])
     rowp = np.mean([ev.ms(stage[4 * k + 2], stage[4 * k + 3]) for k in range(K)])
     ev.destroy(stage)
-    ms_step = total_ms / K
-    host_bound = enqueue_ms > hold_ms
     # the same K transforms replayed from a captured graph (no events)
     graph = cu.capture(lambda: [plan.adjoint(yd, out, stream=stream) for _ in range(K)], stream)
     graph.launch(stream)
