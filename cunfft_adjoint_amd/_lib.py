@@ -27,6 +27,7 @@ CUNA_MODE_ACCURATE = 0
 CUNA_MODE_COMPAT = 1
 CUNA_PLAN_WINDOW2 = 0x1
 CUNA_PLAN_UNFUSED = 0x2
+CUNA_PLAN_GATHER = 0x4
 
 # FLAGS (inc/cuna_typedefs.h:110-116 of the reference)
 NO_FLAGS = 0x01

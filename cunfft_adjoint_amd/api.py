@@ -226,8 +226,8 @@ class NFFTPlan:
 
     def __init__(self, x=None, M: int = 0, *, t=None, df: float | None = None, offsets=None,
                  mode: str = "accurate", sigma: int = 2, m: int | None = None,
-                 tile_log2: int = 0, window2: bool = False, unfused: bool = False, dtype=None,
-                 fft_w: int = 0, fft_h: int = 0, stream=None):
+                 tile_log2: int = 0, window2: bool = False, unfused: bool = False,
+                 gather: bool = False, dtype=None, fft_w: int = 0, fft_h: int = 0, stream=None):
         import torch
 
         if x is not None:
@@ -243,8 +243,9 @@ class NFFTPlan:
             p.m = m
         p.tile_log2 = tile_log2
         p.fft_w, p.fft_h = fft_w, fft_h
-        p.flags = (_lib.CUNA_PLAN_WINDOW2 if window2 else 0) | (
-            _lib.CUNA_PLAN_UNFUSED if unfused else 0)
+        p.flags = ((_lib.CUNA_PLAN_WINDOW2 if window2 else 0)
+                   | (_lib.CUNA_PLAN_UNFUSED if unfused else 0)
+                   | (_lib.CUNA_PLAN_GATHER if gather else 0))
         h = ctypes.c_void_p()
         s = _stream(stream)
         if t is not None:

@@ -209,6 +209,7 @@ SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
     A.periodic = P.periodic ? 1 : 0;
     A.wrap1 = (P.periodic && P.ntiles == 1) ? 1 : 0;
     A.cap = sizeof(T) == 4 ? 2048 : 1024;
+    A.gather = (spec.flags & CUNA_PLAN_GATHER) ? 1 : 0;
     A.bi = T(1) / beta;
     A.wscale = wscale;
     for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
@@ -860,6 +861,8 @@ static ModeSpec spec_from(const cuna_nfft_params *params) {
     require(s.fft_w >= 0 && s.fft_w <= 8 && s.fft_h >= 0 && s.fft_h <= 4,
             "fft_w must be in [0, 8] and fft_h in [0, 4]");
     require(s.mode == CUNA_MODE_ACCURATE || s.mode == CUNA_MODE_COMPAT, "unknown mode");
+    require((s.flags & ~(CUNA_PLAN_WINDOW2 | CUNA_PLAN_UNFUSED | CUNA_PLAN_GATHER)) == 0,
+            "unknown plan flags");
     require(s.tile_log2 == 0 || (s.tile_log2 >= 6 && s.tile_log2 <= 14), "tile_log2 out of range");
     return s;
 }

@@ -142,10 +142,16 @@ __device__ __forceinline__ int lds_sw(int a) { return a ^ ((a >> 4) & 31); }
 // lds[lds_sw(e*W + w)].  Pass 0 reads its inputs through `ld(e, w)`; every pass writes
 // LDS; the routine returns with natural-order results in LDS after a barrier.
 // tw[e] = exp(+2 pi i e / L) (conjugated for S = -1).
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+// `hook` runs once, after pass 0 has issued its loads and before it writes
+// LDS (used to stage a twiddle table whose loads were issued earlier).
 template <typename T, int S, int LG, int W, int NT, int P, int LOGNS, class Loader,
-          bool INPLACE = false>
+          bool INPLACE = false, class Hook = NoHook>
 __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict__ tw,
-                                         const Loader &ld) {
+                                         const Loader &ld, const Hook &hook = Hook()) {
     constexpr int NP = num_passes(LG);
     if constexpr (P < NP) {
         constexpr int LR = pass_log2(LG, P);
@@ -188,6 +194,7 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
                 }
             }
         }
+        if constexpr (P == 0) hook();
         // pass 0 reads through the loader; when that reads this same LDS
         // buffer (INPLACE) all reads must finish before the first write
         if constexpr (P > 0 || INPLACE) __syncthreads();
@@ -212,15 +219,41 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
             }
         }
         __syncthreads();
-        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader, INPLACE>(lds, tw, ld);
+        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader, INPLACE, Hook>(lds, tw, ld, hook);
     }
 }
 
-template <typename T, int S, int LG, int W, int NT, class Loader>
+template <typename T, int S, int LG, int W, int NT, class Loader, class Hook = NoHook>
 __device__ __forceinline__ void lds_fft(cplx<T> *lds, const cplx<T> *__restrict__ tw,
-                                        const Loader &ld) {
-    fft_pass<T, S, LG, W, NT, 0, 0, Loader>(lds, tw, ld);
+                                        const Loader &ld, const Hook &hook = Hook()) {
+    fft_pass<T, S, LG, W, NT, 0, 0, Loader, false, Hook>(lds, tw, ld, hook);
 }
+
+// A twiddle table of L entries copied into LDS: load() issues the global
+// loads early, store() (run as the pass-0 hook) writes them, so the passes
+// read their twiddles from LDS instead of waiting on L2 after each barrier.
+template <typename T, int L, int NT>
+struct TableStage {
+    static constexpr int K = (L + NT - 1) / NT;
+    cplx<T> v[K];
+    __device__ __forceinline__ void load(const cplx<T> *__restrict__ g, int tid) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = tid + k * NT;
+            if (L % NT == 0 || i < L) v[k] = g[i];
+        }
+    }
+    __device__ __forceinline__ void store(cplx<T> *dst, int tid) const {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = tid + k * NT;
+            if (L % NT == 0 || i < L) dst[i] = v[k];
+        }
+    }
+};
+
+// LDS twiddle tables pay off where a pass's table is small next to its data
+constexpr bool lds_table(int lg) { return lg <= 12; }
 
 // the data is already in `lds` in natural (unswizzled) order; transformed in
 // place, the result is in the swizzled layout
@@ -348,6 +381,8 @@ __device__ __forceinline__ void emit_pair(const Sink &sk, cplx<T> *o, long r, lo
 // Forward: the real grid read as complex z[n] (batch stride N).
 template <typename T>
 struct LoadGrid {
+    // plain loads: the kernel can afford to hold staged twiddles in registers
+    static constexpr bool kStage = true;
     const cplx<T> *z;
     long stride;  // per batch
     __device__ __forceinline__ void advance(long b) { z += b * stride; }
@@ -359,6 +394,7 @@ struct LoadGrid {
 // H_f = (c_f + conj c_{G-f})/2, c_f = D(k) g[M/2 + k] for f = k mod G in band.
 template <typename T>
 struct LoadHermitian {
+    static constexpr bool kStage = false;  // heavy loader: twiddles stay in global memory
     const cplx<T> *g;  // M values, natural order k = -M/2..M/2-1
     long gstride;      // per batch
     long M, G, N;
@@ -391,28 +427,69 @@ struct LoadHermitian {
 };
 
 // ------------------------------------------------------------- kernels --
+// Every kernel issues its twiddle loads together with its data loads: the
+// per-pass table goes to LDS (when small, lds_table) through the pass-0 hook,
+// and the epilogue's twiddles are fetched into registers up front, so the
+// workgroup waits on global memory once instead of once per pass.
+// Launch LDS: (sequences + (lds_table(LG) ? 1 : 0)) * 2^LG complex values.
+
+// Two-level twiddle w(e) = lo[e mod B] * hi[e / B], split into its two loads
+// (issued early) and the product (formed late).
+template <typename T>
+struct Tw2Parts {
+    cplx<T> lo, hi;
+    __device__ __forceinline__ void load(const Tw2<T> &t, long e) {
+        lo = t.lo[e & ((1L << t.lb) - 1)];
+        hi = t.hi[e >> t.lb];
+    }
+    __device__ __forceinline__ cplx<T> get() const { return lo * hi; }
+};
+
 // One workgroup per transform (batch = blockIdx.x), whole sequence in LDS.
 // MODE 0: real post-processing into a sink; MODE 1: plain store of z.
 template <typename T, int S, int LG, int NT, class Loader, class Sink, int MODE>
 __global__ __launch_bounds__(NT) void k_fft_small(Loader ld, Sink sk, cplx<T> *plain_out,
                                                   const cplx<T> *tw, Tw2<T> twG) {
-    constexpr long N = 1L << LG;
+    constexpr int N = 1 << LG;
+    constexpr bool TAB = Loader::kStage && lds_table(LG);
+    constexpr int OPT = MODE == 0 ? (N / 2) / NT + 1 : 1;  // epilogue pairs per thread
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
+    cplx<T> *twl = lds + N;
+    const int tid = threadIdx.x;
     const long b = blockIdx.x;
     Loader lb = ld;
     lb.advance(b);
+    TableStage<T, (TAB ? N : 1), NT> tab;
+    if constexpr (TAB) tab.load(tw, tid);
+    Tw2Parts<T> ot[OPT];
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int r = tid + i * NT;
+            if (r <= N / 2) ot[i].load(twG, r);
+        }
+    }
     auto fetch = [&](int e, int) { return lb.at(e); };
-    lds_fft<T, S, LG, 1, NT>(lds, tw, fetch);
+    if constexpr (TAB) {
+        auto hook = [&]() { tab.store(twl, tid); };
+        lds_fft<T, S, LG, 1, NT>(lds, twl, fetch, hook);
+    } else {
+        lds_fft<T, S, LG, 1, NT>(lds, tw, fetch);
+    }
     if constexpr (MODE == 0) {
         cplx<T> *o = sk.out + b * sk.ostride;
-        for (long r = threadIdx.x; r <= N / 2; r += NT) {
-            cplx<T> Zr = lds[lds_sw((int)r)];
-            cplx<T> Zn = lds[lds_sw((int)((N - r) & (N - 1)))];
-            emit_pair(sk, o, r, N, Zr, Zn, twG(r));
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int r = tid + i * NT;
+            if (r <= N / 2) {
+                cplx<T> Zr = lds[lds_sw(r)];
+                cplx<T> Zn = lds[lds_sw((N - r) & (N - 1))];
+                emit_pair(sk, o, (long)r, (long)N, Zr, Zn, ot[i].get());
+            }
         }
     } else {
         cplx<T> *o = plain_out + b * N;
-        for (long e = threadIdx.x; e < N; e += NT) o[e] = lds[lds_sw((int)e)];
+        for (int e = tid; e < N; e += NT) o[e] = lds[lds_sw(e)];
     }
 }
 
@@ -423,21 +500,56 @@ template <typename T, int S, int LG2, int W, int NT, class Loader>
 __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long dst_stride, int L1,
                                                 const cplx<T> *tw, Tw2<T> twN) {
     constexpr int L2 = 1 << LG2;
+    constexpr bool TAB = Loader::kStage && lds_table(LG2);
+    // epilogue twiddles held through the FFT: only where registers are ample
+    constexpr bool PRE = Loader::kStage && NT <= 256;
+    constexpr int OPT = PRE ? (W * L2 + NT - 1) / NT : 1;
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
+    cplx<T> *twl = lds + W * L2;
+    const int tid = threadIdx.x;
     const int nblk = L1 / W;
     const int blk = xcd_remap(blockIdx.x, nblk);
     const long b = blockIdx.y;
     const int n10 = blk * W;
     Loader lb = ld;
     lb.advance(b);
+    TableStage<T, (TAB ? L2 : 1), NT> tab;
+    if constexpr (TAB) tab.load(tw, tid);
+    Tw2Parts<T> ot[OPT];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int idx = tid + i * NT;
+            if ((W * L2) % NT == 0 || idx < W * L2)
+                ot[i].load(twN, (long)(n10 + idx % W) * (idx / W));
+        }
+    }
     auto fetch = [&](int e, int w) { return lb.at(n10 + w + (long)L1 * e); };
-    lds_fft<T, S, LG2, W, NT>(lds, tw, fetch);
+    if constexpr (TAB) {
+        auto hook = [&]() { tab.store(twl, tid); };
+        lds_fft<T, S, LG2, W, NT>(lds, twl, fetch, hook);
+    } else {
+        lds_fft<T, S, LG2, W, NT>(lds, tw, fetch);
+    }
     cplx<T> *o = dst + b * dst_stride;
-    for (int idx = threadIdx.x; idx < W * L2; idx += NT) {
-        const int w = idx % W, r2 = idx / W;
-        cplx<T> t = twN((long)(n10 + w) * r2);
-        if constexpr (S < 0) t.y = -t.y;
-        o[n10 + w + (long)L1 * r2] = lds[lds_sw(r2 * W + w)] * t;
+    if constexpr (PRE) {
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int idx = tid + i * NT;
+            if ((W * L2) % NT == 0 || idx < W * L2) {
+                const int w = idx % W, r2 = idx / W;
+                cplx<T> t = ot[i].get();
+                if constexpr (S < 0) t.y = -t.y;
+                o[n10 + w + (long)L1 * r2] = lds[lds_sw(r2 * W + w)] * t;
+            }
+        }
+    } else {
+        for (int idx = tid; idx < W * L2; idx += NT) {
+            const int w = idx % W, r2 = idx / W;
+            cplx<T> t = twN((long)(n10 + w) * r2);
+            if constexpr (S < 0) t.y = -t.y;
+            o[n10 + w + (long)L1 * r2] = lds[lds_sw(r2 * W + w)] * t;
+        }
     }
 }
 
@@ -451,12 +563,27 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
                                                      Tw2<T> twG) {
     constexpr int L1 = 1 << LG1;
     constexpr int W = 2 * H;
+    constexpr bool TAB = lds_table(LG1);
+    constexpr int OPT = (H * L1 + NT - 1) / NT;
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
+    cplx<T> *twl = lds + W * L1;
+    const int tid = threadIdx.x;
     const int nblk = L2 / (2 * H);
     const int i = xcd_remap(blockIdx.x, nblk);
     const long b = blockIdx.y;
     const long N = (long)L1 * L2;
     const cplx<T> *z = src + b * src_stride;
+    TableStage<T, (TAB ? L1 : 1), NT> tab;
+    if constexpr (TAB) tab.load(tw, tid);
+    // epilogue twiddles for r = rho + L2 r1 (row 0 included; its mirror row
+    // L2/2 takes its own below)
+    Tw2Parts<T> ot[OPT];
+#pragma unroll
+    for (int k = 0; k < OPT; ++k) {
+        const int idx = tid + k * NT;
+        if ((H * L1) % NT == 0 || idx < H * L1)
+            ot[k].load(twG, (long)(i * H + idx % H) + (long)L2 * (idx / H));
+    }
     auto row_of = [&](int s) -> int {
         int h = s % H;
         int rho = i * H + h;
@@ -464,24 +591,32 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
         return rho == 0 ? L2 / 2 : L2 - rho;
     };
     auto fetch = [&](int e, int s) { return z[(long)row_of(s) * L1 + e]; };
-    lds_fft<T, 1, LG1, W, NT>(lds, tw, fetch);
+    if constexpr (TAB) {
+        auto hook = [&]() { tab.store(twl, tid); };
+        lds_fft<T, 1, LG1, W, NT>(lds, twl, fetch, hook);
+    } else {
+        lds_fft<T, 1, LG1, W, NT>(lds, tw, fetch);
+    }
     cplx<T> *o = sk.out + b * sk.ostride;
     // pairs: slot h, element r1 of the primary row with its mirror element
-    for (int idx = threadIdx.x; idx < H * L1; idx += NT) {
+#pragma unroll
+    for (int k = 0; k < OPT; ++k) {
+        const int idx = tid + k * NT;
+        if (!((H * L1) % NT == 0 || idx < H * L1)) continue;
         const int h = idx % H, r1 = idx / H;
         const int rho = i * H + h;
         if (rho != 0) {
             long r = rho + (long)L2 * r1;
             cplx<T> Zr = lds[lds_sw(r1 * W + h)];
             cplx<T> Zn = lds[lds_sw((L1 - 1 - r1) * W + (H + h))];
-            emit_pair(sk, o, r, N, Zr, Zn, twG(r));
+            emit_pair(sk, o, r, N, Zr, Zn, ot[k].get());
         } else {
             // row 0 (slot h=0): element r1 pairs with (L1 - r1) mod L1, r1 <= L1/2
             if (r1 <= L1 / 2) {
                 long r = (long)L2 * r1;
                 cplx<T> Zr = lds[lds_sw(r1 * W + 0)];
                 cplx<T> Zn = lds[lds_sw(((L1 - r1) & (L1 - 1)) * W + 0)];
-                emit_pair(sk, o, r, N, Zr, Zn, twG(r));
+                emit_pair(sk, o, r, N, Zr, Zn, ot[k].get());
             }
             // row L2/2 (slot H): element r1 pairs with L1-1-r1, r1 < L1/2
             if (r1 < L1 / 2) {
@@ -501,15 +636,25 @@ __global__ __launch_bounds__(NT) void k_fft_row_plain(const cplx<T> *src, long s
                                                       int L2, cplx<T> *dst, long dst_stride,
                                                       const cplx<T> *tw) {
     constexpr int L1 = 1 << LG1;
+    constexpr bool TAB = lds_table(LG1);
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
+    cplx<T> *twl = lds + H * L1;
+    const int tid = threadIdx.x;
     const int nblk = L2 / H;
     const int i = xcd_remap(blockIdx.x, nblk);
     const long b = blockIdx.y;
     const cplx<T> *z = src + b * src_stride;
+    TableStage<T, (TAB ? L1 : 1), NT> tab;
+    if constexpr (TAB) tab.load(tw, tid);
     auto fetch = [&](int e, int h) { return z[(long)(i * H + h) * L1 + e]; };
-    lds_fft<T, S, LG1, H, NT>(lds, tw, fetch);
+    if constexpr (TAB) {
+        auto hook = [&]() { tab.store(twl, tid); };
+        lds_fft<T, S, LG1, H, NT>(lds, twl, fetch, hook);
+    } else {
+        lds_fft<T, S, LG1, H, NT>(lds, tw, fetch);
+    }
     cplx<T> *o = dst + b * dst_stride;
-    for (int idx = threadIdx.x; idx < H * L1; idx += NT) {
+    for (int idx = tid; idx < H * L1; idx += NT) {
         const int h = idx % H, r1 = idx / H;
         o[(i * H + h) + (long)L2 * r1] = lds[lds_sw(r1 * H + h)];
     }

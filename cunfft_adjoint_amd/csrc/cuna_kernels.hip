@@ -46,7 +46,7 @@ static void run_small(const FftShape &sh, const Loader &ld, const Sink &sk, cplx
     auto &tc = TwiddleCache<T>::get();
     const cplx<T> *tw = tc.fine(LG);
     Tw2<T> twG = tc.two_level(LG + 1);
-    size_t lds = sizeof(cplx<T>) << LG;
+    size_t lds = (sizeof(cplx<T>) << LG) * (lds_table(LG) ? 2 : 1);
     auto k = k_fft_small<T, S, LG, NT, Loader, Sink, MODE>;
     set_lds(k, lds);
     hipLaunchKernelGGL(k, dim3(sh.batch), dim3(NT), lds, s, ld, sk, plain_out, tw, twG);
@@ -58,7 +58,7 @@ static void run_col(const FftShape &sh, const Loader &ld, cplx<T> *dst, int L1, 
     constexpr int NT = fft_threads(LG2, W);
     auto &tc = TwiddleCache<T>::get();
     const long N = 1L << sh.lgN;
-    size_t lds = (sizeof(cplx<T>) << LG2) * W;
+    size_t lds = (sizeof(cplx<T>) << LG2) * (W + (lds_table(LG2) ? 1 : 0));
     auto k = k_fft_col<T, S, LG2, W, NT, Loader>;
     set_lds(k, lds);
     require(L1 % W == 0, "fft column split");
@@ -73,7 +73,7 @@ static void run_row_real(const FftShape &sh, const cplx<T> *src, int L2, const S
     constexpr int NT = fft_threads(LG1, 2 * H);
     auto &tc = TwiddleCache<T>::get();
     const long N = 1L << sh.lgN;
-    size_t lds = (sizeof(cplx<T>) << LG1) * 2 * H;
+    size_t lds = (sizeof(cplx<T>) << LG1) * (2 * H + (lds_table(LG1) ? 1 : 0));
     auto k = k_fft_row_real<T, LG1, H, NT, Sink>;
     set_lds(k, lds);
     require(L2 % (2 * H) == 0, "fft row split");
@@ -88,7 +88,7 @@ static void run_row_plain(const FftShape &sh, const cplx<T> *src, int L2, cplx<T
     constexpr int NT = fft_threads(LG1, H);
     auto &tc = TwiddleCache<T>::get();
     const long N = 1L << sh.lgN;
-    size_t lds = (sizeof(cplx<T>) << LG1) * H;
+    size_t lds = (sizeof(cplx<T>) << LG1) * (H + (lds_table(LG1) ? 1 : 0));
     auto k = k_fft_row_plain<T, S, LG1, H, NT>;
     set_lds(k, lds);
     require(L2 % H == 0, "fft row split");
@@ -361,6 +361,17 @@ void launch_buckets(const SortWork &W, int K, int *bucket, hipStream_t s) {
 // --------------------------------------------------------------- spread --
 template <typename T, int NTAPS, int VM>
 static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        if (!A.gather) {
+            const size_t lds = spread_fx_lds_bytes<T>(A.tlog, NTAPS);
+            require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
+            auto k = k_spread_fx<T, NTAPS, VM>;
+            set_lds(k, lds);
+            hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
+            CUNA_LAUNCH_CHECK();
+            return;
+        }
+    }
     const size_t lds = spread_lds_bytes<T>(A.tlog, NTAPS, A.cap);
     require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
     auto k = k_spread<T, NTAPS, VM>;

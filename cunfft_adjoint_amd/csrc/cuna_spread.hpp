@@ -322,7 +322,8 @@ struct SpreadArgs {
     int G, tlog, ntiles, Kc, olo, ntaps, nrep;
     int periodic;
     int wrap1;  // periodic and a single tile covers the grid
-    int cap;    // points staged in LDS per round
+    int cap;    // points staged in LDS per round (gather form)
+    int gather; // FP32: use the gather form instead of the fixed-point scatter
     T bi;       // 1/beta
     T wscale;   // weight prefactor
     T e3[kMaxTaps];  // exp(-o^2/beta), o = olo + q
@@ -396,6 +397,68 @@ __host__ __device__ inline size_t spread_lds_bytes(int tlog, int ntaps, int cap)
     return b + sizeof(StagedPoint<T>) * (size_t)cap;
 }
 
+// The sorted points a tile's cells depend on, as up to three ranges with the
+// offset that maps their anchors to this tile's cells, in increasing order of
+// mapped anchor: the previous tile's right-edge class, this tile, the next
+// tile's left-edge class.  A tiny periodic grid (taps wrap more than once)
+// instead uses 2S+1 shifted copies of all its points.
+struct TileRanges {
+    int b0, n0, o0, b1, n1, b2, n2, o2, tstart, total, S, G;
+    bool tiny;
+    template <typename T>
+    __device__ __forceinline__ void init(const SpreadArgs<T> &A, int tile, int c, int ntaps) {
+        const int nt = A.ntiles;
+        const int *bk = A.bucket + (long)c * (A.Kc + 1);
+        const int TT = 1 << A.tlog;
+        G = A.G;
+        tstart = tile << A.tlog;
+        tiny = A.wrap1 && A.G < 2 * ntaps;
+        S = tiny ? (ntaps + A.G - 1) / A.G + 1 : 0;
+        b0 = n0 = o0 = b2 = n2 = o2 = 0;
+        b1 = bk[3 * tile];
+        n1 = bk[3 * tile + 3] - b1;
+        if (!tiny) {
+            int prev = tile - 1, next = tile + 1;
+            bool hp = true, hn = true;
+            if (prev < 0) {
+                prev = nt - 1;
+                hp = A.periodic;
+            }
+            if (next >= nt) {
+                next = 0;
+                hn = A.periodic;
+            }
+            if (hp) {
+                b0 = bk[3 * prev + 2];
+                n0 = bk[3 * prev + 3] - b0;
+                o0 = (prev << A.tlog) + TT;
+            }
+            if (hn) {
+                b2 = bk[3 * next];
+                n2 = bk[3 * next + 1] - b2;
+                o2 = (next << A.tlog) - TT;
+            }
+        }
+        total = tiny ? n1 * (2 * S + 1) : n0 + n1 + n2;
+    }
+    // j-th point of the tile: sorted index s and the offset ro (mapped anchor = a - ro)
+    __device__ __forceinline__ void get(int j, int &s, int &ro) const {
+        if (tiny) {
+            s = b1 + j % n1;
+            ro = -(j / n1 - S) * G;
+        } else if (j < n0) {
+            s = b0 + j;
+            ro = o0;
+        } else if (j < n0 + n1) {
+            s = b1 + (j - n0);
+            ro = tstart;
+        } else {
+            s = b2 + (j - n0 - n1);
+            ro = o2;
+        }
+    }
+};
+
 // grid: blockIdx.x = tile, blockIdx.y = slab (curve * nrep + replicate).
 // Output-driven: every cell of the tile is owned by one thread, which sums
 // v_i exp(-(o_i - e_i)^2/beta) over the points i whose taps reach it (o_i =
@@ -414,45 +477,12 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
     off_pts = (off_pts + 15) & ~(size_t)15;
     StagedPoint<T> *pts = reinterpret_cast<StagedPoint<T> *>(cuna_smem + off_pts);
 
-    const int nt = A.ntiles;
-    const int tile = xcd_remap(blockIdx.x, nt);
+    const int tile = xcd_remap(blockIdx.x, A.ntiles);
     const int slab = blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
-    const int *bk = A.bucket + (long)c * (A.Kc + 1);
-    const int tstart = tile << A.tlog;
-
-    // Ranges of sorted points, each with the offset that maps its anchors to
-    // this tile's cells, in increasing order of mapped anchor: the previous
-    // tile's right-edge class, this tile, the next tile's left-edge class.
-    // A tiny periodic grid (taps wrap more than once) instead uses 2S+1
-    // shifted copies of all its points.
-    const bool tiny = A.wrap1 && A.G < 2 * NTAPS;
-    const int S = tiny ? (NTAPS + A.G - 1) / A.G + 1 : 0;
-    int b0 = 0, n0 = 0, o0 = 0, b1 = bk[3 * tile], n1 = bk[3 * tile + 3] - b1, b2 = 0, n2 = 0,
-        o2 = 0;
-    if (!tiny) {
-        int prev = tile - 1, next = tile + 1;
-        bool hp = true, hn = true;
-        if (prev < 0) {
-            prev = nt - 1;
-            hp = A.periodic;
-        }
-        if (next >= nt) {
-            next = 0;
-            hn = A.periodic;
-        }
-        if (hp) {
-            b0 = bk[3 * prev + 2];
-            n0 = bk[3 * prev + 3] - b0;
-            o0 = (prev << A.tlog) + TT;
-        }
-        if (hn) {
-            b2 = bk[3 * next];
-            n2 = bk[3 * next + 1] - b2;
-            o2 = (next << A.tlog) - TT;
-        }
-    }
-    const int total = tiny ? n1 * (2 * S + 1) : n0 + n1 + n2;
+    TileRanges R;
+    R.init(A, tile, c, NTAPS);
+    const int tstart = R.tstart, total = R.total;
 
     for (int g = threadIdx.x; g < TT; g += NT) gacc[g] = T(0);
     for (int base = 0; base < total; base += A.cap) {
@@ -460,21 +490,8 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
         __syncthreads();
         // stage this round's points: anchor in tile coordinates, e, value
         for (int i = threadIdx.x; i < cnt; i += NT) {
-            const int j = base + i;
             int s, ro;
-            if (tiny) {
-                s = b1 + j % n1;
-                ro = -(j / n1 - S) * A.G;
-            } else if (j < n0) {
-                s = b0 + j;
-                ro = o0;
-            } else if (j < n0 + n1) {
-                s = b1 + (j - n0);
-                ro = tstart;
-            } else {
-                s = b2 + (j - n0 - n1);
-                ro = o2;
-            }
+            R.get(base + i, s, ro);
             const PointRec<T> pr = A.rec[s];
             T v, e = pr.e;
             point_value<T, VM>(A, A.perm[s], c, rep, v, e);
@@ -509,6 +526,141 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
     T *gout = A.grid + (long)slab * A.grid_stride + tstart;
     const int tlen = min(TT, A.G - tstart);
     for (int g = threadIdx.x; g < tlen; g += NT) gout[g] = gacc[g] * A.wscale;
+}
+
+// ---------------------------------------------- fixed-point scatter --
+// Scatter form for the FP32 library: a thread takes a point, forms its
+// NTAPS weights with three exponentials, and adds them to the tile's LDS
+// cells with 64-bit integer atomics (ds_add_u64) in fixed point.  Integer
+// sums do not depend on the order of the additions, so the result is
+// bitwise reproducible; float LDS atomics would be neither reproducible nor
+// fast on this hardware.  A round of up to CAP points uses the scale 2^s
+// with cnt * max|v| * wscale * 2^s <= 2^62, so no cell can overflow, and a
+// weight keeps at least 2^-37 of the round's largest weight as resolution
+// (float itself keeps 2^-24 of the cell's own sum).
+// Split of a scaled weight x (|x| < 2^62) into a two's-complement 64-bit
+// integer: h = floor(x / 2^32) and l = x - h 2^32 in [0, 2^32).  Exact for
+// x >= 0; for -2^31 < x < 0, l is rounded to a multiple of 2^8, an error far
+// below the float weight's own rounding.
+__device__ __forceinline__ unsigned long long to_fixed64(float x) {
+    const float h = floorf(x * 0x1p-32f);
+    const float l = fminf(fmaf(h, -0x1p32f, x), 4294967040.0f);
+    return ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)l;
+}
+
+template <typename T, int NTAPS, int VM>
+__global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
+    static_assert(sizeof(T) == 4, "fixed-point scatter is the FP32 spread");
+    constexpr int NT = 256, PPT = 4, CAP = NT * PPT, Q0 = (NTAPS - 1) / 2, HALO = NTAPS;
+    const int TT = 1 << A.tlog;
+    const int NC = TT + 2 * HALO;  // cells -HALO .. TT+HALO-1
+    unsigned long long *acc = reinterpret_cast<unsigned long long *>(cuna_smem);
+    float *facc = reinterpret_cast<float *>(acc + NC);  // TT, for tiles of several rounds
+    __shared__ float red[NT / 64];
+
+    const int tile = xcd_remap(blockIdx.x, A.ntiles);
+    const int slab = blockIdx.y;
+    const int c = slab / A.nrep, rep = slab % A.nrep;
+    TileRanges R;
+    R.init(A, tile, c, NTAPS);
+    const int total = R.total;
+    const bool multi = total > CAP;
+    const int tid = threadIdx.x;
+
+    for (int g = tid; g < NC; g += NT) acc[g] = 0ull;
+    if (multi)
+        for (int g = tid; g < TT; g += NT) facc[g] = 0.f;
+    bool finite = true;
+    int sexp = 0;
+    for (int base = 0; base < total; base += CAP) {
+        const int cnt = min(CAP, total - base);
+        int am[PPT];
+        float ev[PPT], vv[PPT];
+        float vmax = 0.f;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int j = base + k * NT + tid;
+            am[k] = 0;
+            ev[k] = 0.f;
+            vv[k] = 0.f;
+            if (j < base + cnt) {
+                int s, ro;
+                R.get(j, s, ro);
+                const PointRec<T> pr = A.rec[s];
+                T v, e = pr.e;
+                point_value<T, VM>(A, A.perm[s], c, rep, v, e);
+                am[k] = pr.a - ro;
+                ev[k] = e;
+                vv[k] = v;
+                // a non-finite value or position (NaN x gives NaN e) makes
+                // the round's maximum infinite, and the tile's output NaN
+                float av = fabsf(v);
+                if (!(av <= 3.0e38f) || !(fabsf(e) <= 3.0e38f)) av = INFINITY;
+                vmax = fmaxf(vmax, av);
+            }
+        }
+        // round maximum of |v|
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+        if ((tid & 63) == 0) red[tid >> 6] = vmax;
+        __syncthreads();
+        vmax = red[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) vmax = fmaxf(vmax, red[w]);
+        if (!(vmax <= 3.0e38f)) finite = false;
+        // 2^sexp: cnt * vmax * wscale * 2^sexp <= 2^62, and wscale * 2^sexp <= 2^100
+        const float bound = (float)cnt * vmax * A.wscale;
+        int eb = 0;
+        if (bound > 0.f && finite) (void)frexpf(bound, &eb);
+        else eb = -64;
+        int ew = 0;
+        (void)frexpf(A.wscale, &ew);
+        sexp = min(62 - eb, 100 - ew);
+        const float mul = ldexpf(A.wscale, sexp);
+        if (finite) {
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (base + k * NT + tid < base + cnt) {
+                    float w[NTAPS];
+                    tap_weights<float, NTAPS>(vv[k] * mul, ev[k], A.bi, A.e3, w);
+                    const int c0 = am[k] - Q0;  // first tap's cell
+                    if (R.tiny) {
+#pragma unroll
+                        for (int q = 0; q < NTAPS; ++q) {
+                            const int cell = c0 + q;
+                            if (cell >= -HALO && cell < TT + HALO)
+                                atomicAdd(&acc[HALO + cell], to_fixed64(w[q]));
+                        }
+                    } else {
+                        unsigned long long *dst = acc + HALO + c0;
+#pragma unroll
+                        for (int q = 0; q < NTAPS; ++q) atomicAdd(dst + q, to_fixed64(w[q]));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (multi) {
+            const float inv = ldexpf(1.f, -sexp);
+            for (int g = tid; g < TT; g += NT)
+                facc[g] += (float)(long long)acc[HALO + g] * inv;
+            for (int g = tid; g < NC; g += NT) acc[g] = 0ull;
+            __syncthreads();
+        }
+    }
+    T *gout = A.grid + (long)slab * A.grid_stride + R.tstart;
+    const int tlen = min(TT, A.G - R.tstart);
+    const float inv = ldexpf(1.f, -sexp);
+    for (int g = tid; g < tlen; g += NT) {
+        float v = multi ? facc[g] : (float)(long long)acc[HALO + g] * inv;
+        gout[g] = finite ? v : NAN;
+    }
+}
+
+template <typename T>
+__host__ __device__ inline size_t spread_fx_lds_bytes(int tlog, int ntaps) {
+    const size_t tt = (size_t)1 << tlog;
+    return sizeof(unsigned long long) * (tt + 2 * (size_t)ntaps) + sizeof(float) * tt;
 }
 
 // ------------------------------------------- fused batched bootstrap --

@@ -220,6 +220,41 @@ def test_determinism_bitwise():
     assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
 
 
+@pytest.mark.parametrize("pts,M,tile", [("clustered", 1 << 16, 0), ("dense", 1 << 12, 8),
+                                         ("uniform", 1 << 14, 6), ("tiny", 8, 0)])
+def test_fp32_scatter_and_gather_forms(pts, M, tile):
+    """The FP32 spread has two forms: fixed-point scatter (default) and gather.
+    Both match the NDFT and each other; "dense" puts ~20k points in one tile
+    (many scatter rounds), "tiny" is a grid smaller than two filter widths."""
+    if pts == "clustered":
+        x, y = clustered_points(300_000, 12)
+    elif pts == "dense":
+        rng = np.random.default_rng(13)
+        x = rng.uniform(0.1, 0.1 + 256 / (2 * M), 20_000).astype(np.float32)
+        y = rng.standard_normal(20_000).astype(np.float32)
+    else:
+        x, y = uniform_points(5000 if pts == "tiny" else 100_000, 14)
+    ref = oracle.ndft(x.astype(np.float64), y.astype(np.float64), M)
+    outs = []
+    for gather in (False, True):
+        plan = cu.NFFTPlan(dev(x), M, gather=gather, tile_log2=tile)
+        a = host(plan.adjoint(dev(y)))[0]
+        b = host(plan.adjoint(dev(y)))[0]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert rel_l2(a, ref) < 1e-5, (gather, rel_l2(a, ref))
+        outs.append(a)
+    assert rel_l2(outs[0], outs[1]) < 2e-6
+
+
+def test_fp32_nonfinite_value_propagates():
+    x, y = uniform_points(5000, 15)
+    y = y.copy()
+    y[1234] = np.nan
+    for gather in (False, True):
+        out = host(cu.NFFTPlan(dev(x), 1 << 10, gather=gather).adjoint(dev(y)))[0]
+        assert np.isnan(out).all()
+
+
 def test_batch_equals_single_curves():
     rng = np.random.default_rng(3)
     lens = [0, 1, 100, 5000, 20000, 333]
