@@ -148,10 +148,11 @@ struct NoHook {
 
 // `hook` runs once, after pass 0 has issued its loads and before it writes
 // LDS (used to stage a twiddle table whose loads were issued earlier).
+// `tw` is a table pointer or any object with operator[](int) giving w_L^e.
 template <typename T, int S, int LG, int W, int NT, int P, int LOGNS, class Loader,
-          bool INPLACE = false, class Hook = NoHook>
-__device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict__ tw,
-                                         const Loader &ld, const Hook &hook = Hook()) {
+          bool INPLACE = false, class Hook = NoHook, class TW = const cplx<T> *>
+__device__ __forceinline__ void fft_pass(cplx<T> *lds, TW tw, const Loader &ld,
+                                         const Hook &hook = Hook()) {
     constexpr int NP = num_passes(LG);
     if constexpr (P < NP) {
         constexpr int LR = pass_log2(LG, P);
@@ -219,7 +220,7 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, const cplx<T> *__restrict
             }
         }
         __syncthreads();
-        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader, INPLACE, Hook>(lds, tw, ld, hook);
+        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader, INPLACE, Hook, TW>(lds, tw, ld, hook);
     }
 }
 
@@ -257,10 +258,10 @@ constexpr bool lds_table(int lg) { return lg <= 12; }
 
 // the data is already in `lds` in natural (unswizzled) order; transformed in
 // place, the result is in the swizzled layout
-template <typename T, int S, int LG, int NT>
-__device__ __forceinline__ void lds_fft_inplace(cplx<T> *lds, const cplx<T> *__restrict__ tw) {
+template <typename T, int S, int LG, int NT, class TW>
+__device__ __forceinline__ void lds_fft_inplace(cplx<T> *lds, TW tw) {
     auto fetch = [&](int e, int) { return lds[e]; };
-    fft_pass<T, S, LG, 1, NT, 0, 0, decltype(fetch), true>(lds, tw, fetch);
+    fft_pass<T, S, LG, 1, NT, 0, 0, decltype(fetch), true, NoHook, TW>(lds, tw, fetch);
 }
 
 // ------------------------------------------------------ twiddle tables --
@@ -274,6 +275,11 @@ struct Tw2 {
     __device__ __forceinline__ cplx<T> operator()(long e) const {
         return lo[e & ((1L << lb) - 1)] * hi[e >> lb];
     }
+    __device__ __forceinline__ cplx<T> operator[](int e) const {
+        return lo[e & ((1 << lb) - 1)] * hi[e >> lb];
+    }
+    // entries of the two tables
+    __host__ __device__ int n_lo() const { return 1 << lb; }
 };
 
 // Device twiddle tables, generated in long double on the host once per size

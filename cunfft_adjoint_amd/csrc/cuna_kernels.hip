@@ -449,9 +449,10 @@ template <typename T, int LG, int NTAPS>
 static void boot_fused_one(BootArgs<T> A, hipStream_t s) {
     constexpr int NT = fft_threads(LG, 1) < 256 ? 256 : fft_threads(LG, 1);
     auto &tc = TwiddleCache<T>::get();
-    A.tw = tc.fine(LG);
+    A.twN = tc.two_level(LG);
     A.twG = tc.two_level(LG + 1);
-    const size_t lds = sizeof(cplx<T>) << LG;
+    const size_t lds = boot_lds_bytes<T>(LG);
+    static_assert(boot_lds_bytes<T>(LG) <= kMaxLds, "fused bootstrap LDS");
     auto k = k_boot_fused<T, LG, NTAPS, NT>;
     set_lds(k, lds);
     int dev = 0, ncu = 0;

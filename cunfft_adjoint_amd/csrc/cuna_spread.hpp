@@ -44,6 +44,28 @@ __device__ __host__ __forceinline__ uint32_t xorshift32(uint32_t s) {
     return s;
 }
 
+// x mod d for 32-bit unsigned x and a divisor 1 <= d < 2^31 shared by many x,
+// exact (Granlund & Montgomery 1994, unsigned division by a run-time
+// invariant): l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// q = (t + ((x - t) >> min(l,1))) >> max(l-1,0) with t = mulhi(m, x).
+struct FastMod {
+    uint32_t d, m;
+    int s1, s2;
+    __device__ __host__ void init(uint32_t d_) {
+        d = d_;
+        int l = 0;
+        while ((1ull << l) < d) ++l;
+        m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+        s1 = l < 1 ? l : 1;
+        s2 = l > 1 ? l - 1 : 0;
+    }
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+        const uint32_t t = __umulhi(m, x);
+        const uint32_t q = (t + ((x - t) >> s1)) >> s2;
+        return x - q * d;
+    }
+};
+
 // ------------------------------------------------------ point setup --
 enum SetupMode { SETUP_COMPAT = 0, SETUP_ACC_X = 1, SETUP_ACC_T = 2 };
 
@@ -477,7 +499,9 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
     off_pts = (off_pts + 15) & ~(size_t)15;
     StagedPoint<T> *pts = reinterpret_cast<StagedPoint<T> *>(cuna_smem + off_pts);
 
-    const int tile = xcd_remap(blockIdx.x, A.ntiles);
+    // consecutive tiles go to different XCDs (hardware round-robin): dense
+    // regions of clustered points are spread over all of them
+    const int tile = blockIdx.x;
     const int slab = blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
     TileRanges R;
@@ -558,7 +582,9 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
     float *facc = reinterpret_cast<float *>(acc + NC);  // TT, for tiles of several rounds
     __shared__ float red[NT / 64];
 
-    const int tile = xcd_remap(blockIdx.x, A.ntiles);
+    // consecutive tiles go to different XCDs (hardware round-robin): dense
+    // regions of clustered points are spread over all of them
+    const int tile = blockIdx.x;
     const int slab = blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
     TileRanges R;
@@ -695,9 +721,21 @@ struct BootArgs {
     T bi, wscale;
     T e3[kMaxTaps];
     SinkAccurate<T> sink;       // out pointer set per item
-    const cplx<T> *tw;
-    Tw2<T> twG;
+    Tw2<T> twN;                 // w_N^e, e < N (two-level, staged into LDS)
+    Tw2<T> twG;                 // w_G^e, e < G (two-level, staged into LDS)
 };
+
+// LDS of the fused kernel: the grid (G reals = N complex) and the four
+// twiddle sub-tables (Tw2 of N and of G).
+template <typename T>
+__host__ __device__ constexpr int boot_tw_entries(int lg) {
+    return (1 << ((lg + 1) / 2)) + (1 << (lg - (lg + 1) / 2)) + (1 << ((lg + 2) / 2)) +
+           (1 << (lg + 1 - (lg + 2) / 2));
+}
+template <typename T>
+__host__ __device__ constexpr size_t boot_lds_bytes(int lg) {
+    return sizeof(cplx<T>) * (((size_t)1 << lg) + (size_t)boot_tw_entries<T>(lg));
+}
 
 static __global__ void k_reset_counter(int *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
@@ -724,13 +762,29 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
     using U = std::make_unsigned_t<I>;
     static_assert(sizeof(I) == sizeof(T), "fixed-point grid aliases the real grid");
     constexpr int N = 1 << LG, G = 2 * N, Q0 = (NTAPS - 1) / 2;
+    constexpr int LBN = (LG + 1) / 2, LBG = (LG + 2) / 2;  // as TwiddleCache::two_level
+    constexpr int NLN = 1 << LBN, NHN = 1 << (LG - LBN), NLG = 1 << LBG, NHG = 1 << (LG + 1 - LBG);
+    constexpr int PB = 4;  // resampled points per thread whose loads are in flight together
     constexpr int kMaxExp = sizeof(T) == 4 ? 100 : 1000;
     T *grid = reinterpret_cast<T *>(cuna_smem);
     U *gu = reinterpret_cast<U *>(cuna_smem);
     cplx<T> *z = reinterpret_cast<cplx<T> *>(cuna_smem);
+    cplx<T> *tl = z + N;  // twiddle sub-tables, staged once per workgroup
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NLN; i += NT) tl[i] = A.twN.lo[i];
+    for (int i = tid; i < NHN; i += NT) tl[NLN + i] = A.twN.hi[i];
+    for (int i = tid; i < NLG; i += NT) tl[NLN + NHN + i] = A.twG.lo[i];
+    for (int i = tid; i < NHG; i += NT) tl[NLN + NHN + NLG + i] = A.twG.hi[i];
+    Tw2<T> twN, twG;
+    twN.lo = tl;
+    twN.hi = tl + NLN;
+    twN.lb = LBN;
+    twG.lo = tl + NLN + NHN;
+    twG.hi = tl + NLN + NHN + NLG;
+    twG.lb = LBG;
     __shared__ int item_sh, overflow_sh;
     for (;;) {
-        if (threadIdx.x == 0) item_sh = atomicAdd(A.counter, 1);
+        if (tid == 0) item_sh = atomicAdd(A.counter, 1);
         __syncthreads();
         const int item = item_sh;
         if (item >= A.total) break;  // uniform over the workgroup
@@ -743,6 +797,8 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
         const PointRec<T> *P = tr == 0 ? A.rec_x : A.rec_2x;
         const bool usey = tr == 0 && A.y;
         const double vm = usey ? (double)A.ymax[ci] : 1.0;
+        FastMod fm;
+        fm.init((uint32_t)max(nc, 1));
         int sexp = 0;
         if (vm <= 1e300) {  // false for NaN and inf
             // first attempt: 4x the densest window of the original points;
@@ -754,27 +810,46 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
                 const int e = bound > 0 ? (int)ceil(log2(bound)) : -kMaxExp;
                 sexp = min(max(FixedOf<T>::headroom_bits - e, -kMaxExp), kMaxExp);
                 const T mul = (T)ldexp((double)A.wscale, sexp);
-                for (int i = threadIdx.x; i < G; i += NT) gu[i] = U(0);
-                if (threadIdx.x == 0) overflow_sh = 0;
+                for (int i = tid; i < G; i += NT) gu[i] = U(0);
+                if (tid == 0) overflow_sh = 0;
                 __syncthreads();
-                bool ovf = false;
-                for (int p = threadIdx.x; p < nc; p += NT) {
-                    const uint32_t t = (uint32_t)rep * (uint32_t)nc + (uint32_t)p;
-                    const int d = c0 + (int)(xorshift32(wang_hash(A.seed + t)) % (uint32_t)nc);
-                    const PointRec<T> pr = P[d];
-                    const T v = usey ? A.y[d] : T(1);
-                    T w[NTAPS];
-                    tap_weights<T, NTAPS>(v * mul, pr.e, A.bi, A.e3, w);
-                    const int base = pr.a - Q0;
+                // With cnt >= 4 every addition is below 2^(bits-4) in size, so a
+                // cell can only wrap after a returned value reached 2^(bits-2) in
+                // size: adding 2^(bits-2) sets the top bit exactly for those.
+                constexpr U kQuarter = U(1) << (8 * sizeof(U) - 2);
+                U flag = 0;
+                for (int p0 = 0; p0 < nc; p0 += NT * PB) {
+                    PointRec<T> pr[PB];
+                    T vv[PB];
 #pragma unroll
-                    for (int q = 0; q < NTAPS; ++q) {
-                        const I add = FixedOf<T>::round(w[q]);
-                        const I old = (I)atomicAdd(&gu[(base + q) & (G - 1)], (U)add);
-                        const I nw = (I)((U)old + (U)add);
-                        ovf |= ((old ^ nw) & (add ^ nw)) < 0;  // signed wrap
+                    for (int k = 0; k < PB; ++k) {
+                        const int p = p0 + k * NT + tid;
+                        pr[k].a = 0;
+                        pr[k].e = T(0);
+                        vv[k] = T(0);
+                        if (p < nc) {
+                            const uint32_t t = (uint32_t)rep * (uint32_t)nc + (uint32_t)p;
+                            const int d = c0 + (int)fm(xorshift32(wang_hash(A.seed + t)));
+                            pr[k] = P[d];
+                            vv[k] = usey ? A.y[d] : T(1);
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < PB; ++k) {
+                        if (p0 + k * NT + tid < nc) {
+                            T w[NTAPS];
+                            tap_weights<T, NTAPS>(vv[k] * mul, pr[k].e, A.bi, A.e3, w);
+                            const int base = pr[k].a - Q0;
+#pragma unroll
+                            for (int q = 0; q < NTAPS; ++q) {
+                                const I add = FixedOf<T>::round(w[q]);
+                                const U old = atomicAdd(&gu[(base + q) & (G - 1)], (U)add);
+                                flag |= old + kQuarter;
+                            }
+                        }
                     }
                 }
-                if (ovf) overflow_sh = 1;
+                if (flag >> (8 * sizeof(U) - 1)) overflow_sh = 1;
                 __syncthreads();
                 const bool redo = overflow_sh != 0;
                 __syncthreads();
@@ -782,17 +857,17 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
             }
             // fixed point -> real, in place
             const T inv = (T)ldexp(1.0, -sexp);
-            for (int i = threadIdx.x; i < G; i += NT) grid[i] = (T)(I)gu[i] * inv;
+            for (int i = tid; i < G; i += NT) grid[i] = (T)(I)gu[i] * inv;
         } else {
-            for (int i = threadIdx.x; i < G; i += NT) grid[i] = (T)NAN;
+            for (int i = tid; i < G; i += NT) grid[i] = (T)NAN;
         }
         __syncthreads();
-        lds_fft_inplace<T, 1, LG, NT>(z, A.tw);
+        lds_fft_inplace<T, 1, LG, NT>(z, twN);
         SinkAccurate<T> sk = A.sink;
         cplx<T> *o = (tr == 0 ? A.out_y : A.out_w) + ((long)ci * A.nbs + rep) * sk.M;
-        for (int r = threadIdx.x; r <= N / 2; r += NT)
+        for (int r = tid; r <= N / 2; r += NT)
             emit_pair(sk, o, (long)r, (long)N, z[lds_sw(r)], z[lds_sw((N - r) & (N - 1))],
-                      A.twG(r));
+                      twG(r));
         __syncthreads();
     }
 }
