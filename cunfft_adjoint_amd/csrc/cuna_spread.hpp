@@ -463,21 +463,22 @@ struct TileRanges {
         }
         total = tiny ? n1 * (2 * S + 1) : n0 + n1 + n2;
     }
-    // j-th point of the tile: sorted index s and the offset ro (mapped anchor = a - ro)
+    // j-th point of the tile: sorted index s and the offset ro (mapped anchor
+    // = a - ro); selects rather than branches, so that loads indexed by it
+    // stay in one basic block and issue together
     __device__ __forceinline__ void get(int j, int &s, int &ro) const {
-        if (tiny) {
+        if (tiny) {  // uniform
             s = b1 + j % n1;
             ro = -(j / n1 - S) * G;
-        } else if (j < n0) {
-            s = b0 + j;
-            ro = o0;
-        } else if (j < n0 + n1) {
-            s = b1 + (j - n0);
-            ro = tstart;
-        } else {
-            s = b2 + (j - n0 - n1);
-            ro = o2;
+            return;
         }
+        // (selects between computed values only: a select between two
+        // fields would become a select of their addresses and keep this
+        // struct out of registers)
+        const bool in0 = j < n0, in1 = j < n0 + n1;
+        const int s0 = b0 + j, s1 = b1 + (j - n0), s2 = b2 + (j - n0 - n1);
+        s = in0 ? s0 : (in1 ? s1 : s2);
+        ro = o2 + (in1 ? tstart - o2 : 0) + (in0 ? o0 - tstart : 0);
     }
 };
 
@@ -512,18 +513,35 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
     for (int base = 0; base < total; base += A.cap) {
         const int cnt = min(A.cap, total - base);
         __syncthreads();
-        // stage this round's points: anchor in tile coordinates, e, value
-        for (int i = threadIdx.x; i < cnt; i += NT) {
-            int s, ro;
-            R.get(base + i, s, ro);
-            const PointRec<T> pr = A.rec[s];
-            T v, e = pr.e;
-            point_value<T, VM>(A, A.perm[s], c, rep, v, e);
-            StagedPoint<T> sp;
-            sp.a = pr.a - ro;
-            sp.e = e;
-            sp.v = v;
-            pts[i] = sp;
+        // stage this round's points: anchor in tile coordinates, e, value;
+        // four per thread at a time with branch-free loads (slots past the
+        // end read point 0 of the round and are not stored)
+        for (int i0 = 0; i0 < cnt; i0 += 4 * NT) {
+            int ak[4], pk[4];
+            T ek[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + k * NT + threadIdx.x;
+                int s, ro;
+                R.get(base + (i < cnt ? i : 0), s, ro);
+                const PointRec<T> pr = A.rec[s];
+                ak[k] = pr.a - ro;
+                ek[k] = pr.e;
+                pk[k] = A.perm[s];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + k * NT + threadIdx.x;
+                T v, e = ek[k];
+                point_value<T, VM>(A, pk[k], c, rep, v, e);
+                if (i < cnt) {
+                    StagedPoint<T> sp;
+                    sp.a = ak[k];
+                    sp.e = e;
+                    sp.v = v;
+                    pts[i] = sp;
+                }
+            }
         }
         for (int L = threadIdx.x; L < LMAX - LMIN; L += NT) start[L] = cnt;
         __syncthreads();
@@ -600,30 +618,34 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
     int sexp = 0;
     for (int base = 0; base < total; base += CAP) {
         const int cnt = min(CAP, total - base);
-        int am[PPT];
+        // Branch-free loads: a slot past the round's end reads the round's
+        // first point and is masked afterwards, so that all PPT slots'
+        // loads issue together (a branch per slot would wait out each one).
+        int am[PPT], pk[PPT];
         float ev[PPT], vv[PPT];
-        float vmax = 0.f;
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const int j = base + k * NT + tid;
-            am[k] = 0;
-            ev[k] = 0.f;
-            vv[k] = 0.f;
-            if (j < base + cnt) {
-                int s, ro;
-                R.get(j, s, ro);
-                const PointRec<T> pr = A.rec[s];
-                T v, e = pr.e;
-                point_value<T, VM>(A, A.perm[s], c, rep, v, e);
-                am[k] = pr.a - ro;
-                ev[k] = e;
-                vv[k] = v;
-                // a non-finite value or position (NaN x gives NaN e) makes
-                // the round's maximum infinite, and the tile's output NaN
-                float av = fabsf(v);
-                if (!(av <= 3.0e38f) || !(fabsf(e) <= 3.0e38f)) av = INFINITY;
-                vmax = fmaxf(vmax, av);
-            }
+            int s, ro;
+            R.get(j < base + cnt ? j : base, s, ro);
+            const PointRec<T> pr = A.rec[s];
+            am[k] = pr.a - ro;
+            ev[k] = pr.e;
+            pk[k] = A.perm[s];
+        }
+        float vmax = 0.f;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            T v, e = ev[k];
+            point_value<T, VM>(A, pk[k], c, rep, v, e);
+            const bool ok = base + k * NT + tid < base + cnt;
+            vv[k] = ok ? v : 0.f;
+            ev[k] = e;
+            // a non-finite value or position (NaN x gives NaN e) makes the
+            // round's maximum infinite, and the tile's output NaN
+            float av = fabsf(vv[k]);
+            if (!(av <= 3.0e38f) || (ok && !(fabsf(e) <= 3.0e38f))) av = INFINITY;
+            vmax = fmaxf(vmax, av);
         }
         // round maximum of |v|
 #pragma unroll
@@ -821,18 +843,24 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
                 for (int p0 = 0; p0 < nc; p0 += NT * PB) {
                     PointRec<T> pr[PB];
                     T vv[PB];
+                    // branch-free: out-of-range slots draw for p = 0 (and are
+                    // skipped below), so all loads issue together
+                    int dk[PB];
 #pragma unroll
                     for (int k = 0; k < PB; ++k) {
-                        const int p = p0 + k * NT + tid;
-                        pr[k].a = 0;
-                        pr[k].e = T(0);
-                        vv[k] = T(0);
-                        if (p < nc) {
-                            const uint32_t t = (uint32_t)rep * (uint32_t)nc + (uint32_t)p;
-                            const int d = c0 + (int)fm(xorshift32(wang_hash(A.seed + t)));
-                            pr[k] = P[d];
-                            vv[k] = usey ? A.y[d] : T(1);
-                        }
+                        int p = p0 + k * NT + tid;
+                        p = p < nc ? p : 0;
+                        const uint32_t t = (uint32_t)rep * (uint32_t)nc + (uint32_t)p;
+                        dk[k] = c0 + (int)fm(xorshift32(wang_hash(A.seed + t)));
+                    }
+#pragma unroll
+                    for (int k = 0; k < PB; ++k) pr[k] = P[dk[k]];
+                    if (usey) {
+#pragma unroll
+                        for (int k = 0; k < PB; ++k) vv[k] = A.y[dk[k]];
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < PB; ++k) vv[k] = T(1);
                     }
 #pragma unroll
                     for (int k = 0; k < PB; ++k) {
