@@ -304,6 +304,28 @@ class NFFTPlan:
                                                   _ptr(out_y), _ptr(out_w), _stream(stream)))
         return (out_y, out_w) if window else out_y
 
+    def bootstrap_ls(self, y, nbs: int, seed: int = 0, power: bool = True, peak: bool = True,
+                     stream=None, out_power=None, out_peak=None, out_kpeak=None):
+        """Lomb-Scargle periodograms of the bootstrap replicates (plan made
+        with window2=True): power (ncurves*nbs, M/2) for k = 0..M/2-1 and/or
+        peak (ncurves*nbs,) = max over k >= 1 with kpeak its k."""
+        import torch
+
+        rows = self.ncurves * nbs
+        rd = real_dtype(self.prec)
+        if power and out_power is None:
+            out_power = torch.empty((rows, self.M // 2), dtype=rd, device="cuda")
+        if peak and out_peak is None:
+            out_peak = torch.empty(rows, dtype=rd, device="cuda")
+        if peak and out_kpeak is None:
+            out_kpeak = torch.empty(rows, dtype=torch.int32, device="cuda")
+        check(self.L, self.L.cuna_nfft_bootstrap_ls(
+            self.h, _ptr(y), nbs, seed & 0xFFFFFFFF, _ptr(out_power if power else None),
+            _ptr(out_peak if peak else None), _ptr(out_kpeak if peak else None),
+            _stream(stream)))
+        return {"power": out_power if power else None, "peak": out_peak if peak else None,
+                "kpeak": out_kpeak if peak else None}
+
     def backward(self, y, grad_out, want_y: bool = True, want_x: bool = True, stream=None):
         import torch
 

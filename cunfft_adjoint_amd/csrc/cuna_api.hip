@@ -296,6 +296,70 @@ void Plan<T>::prepare_bootstrap(int nbs, bool window) {
     grid.ensure(sizeof(T) * (size_t)G * ncurves() * nbs);
 }
 
+// arguments shared by both modes of the fused bootstrap kernel; computes
+// the per-curve cover (once per plan) and max |y| (every call) on stream s
+template <typename T>
+BootArgs<T> Plan<T>::fused_args(const T *y, int nbs, uint32_t seed, hipStream_t s) {
+    const int nc = ncurves();
+    int *cov = cover.as<int>();
+    if (!cover_ready) {
+        for (int w = 0; w < (has_pts2 ? 2 : 1); ++w) {
+            const TiledPoints<T> &P = w == 0 ? pts : pts2;
+            launch_curve_cover<T>(P.rec_orig.template as<PointRec<T>>(), P.coff.template as<int>(),
+                                  nc, G, ntaps, cov + w * nc, s);
+        }
+        cover_ready = true;
+    }
+    if (y) launch_curve_absmax<T>(y, pts.coff.template as<int>(), nc, ymax.as<T>(), s);
+    BootArgs<T> B{};
+    B.rec_x = pts.rec_orig.template as<PointRec<T>>();
+    B.rec_2x = has_pts2 ? pts2.rec_orig.template as<PointRec<T>>() : nullptr;
+    B.y = y;
+    B.ymax = ymax.as<T>();
+    B.cover_x = cov;
+    B.cover_2x = cov + nc;
+    B.coff = pts.coff.template as<int>();
+    B.order = pts.order.template as<int>();
+    B.counter = counter.as<int>();
+    B.ncurves = nc;
+    B.nbs = nbs;
+    B.seed = seed;
+    B.bi = T(1) / beta;
+    B.wscale = wscale;
+    for (int q = 0; q < kMaxTaps; ++q) B.e3[q] = e3[q];
+    const double pi = 3.14159265358979323846;
+    B.sink.M = M;
+    B.sink.ostride = M;
+    B.sink.invG = T(1) / (T)G;
+    B.sink.coef = (T)(pi * pi * (double)beta);
+    B.sink.pref = (T)(1.0 / std::sqrt(pi * (double)beta));
+    return B;
+}
+
+template <typename T>
+void Plan<T>::bootstrap_ls(const T *y, int nbs, uint32_t seed, T *power, T *peak, int *kpeak,
+                           hipStream_t s) {
+    require(spec.mode == CUNA_MODE_ACCURATE, "Lomb-Scargle bootstrap needs an accurate-mode plan");
+    require(has_pts2, "Lomb-Scargle bootstrap needs a plan made with CUNA_PLAN_WINDOW2");
+    require(nbs > 0, "nbs must be positive");
+    require(y != nullptr, "y is NULL");
+    require(power || peak || kpeak, "no output requested");
+    require(boot_fused_supported<T>(lgN, ntaps),
+            "Lomb-Scargle bootstrap needs a grid of at most 2^15 cells (float) / 2^14 (double) and "
+            "m in {6, 8, 10, 12, 13, 16}",
+            CUNA_ERROR_UNSUPPORTED);
+    prepare_bootstrap(nbs, true);
+    BootArgs<T> B = fused_args(y, nbs, seed, s);
+    B.ls_power = power;
+    B.ls_peak = peak;
+    B.ls_kpeak = kpeak;
+    B.ntr = 1;
+    const long total = (long)ncurves() * nbs;
+    require(total < (1L << 31), "too many bootstrap items");
+    B.total = (int)total;
+    launch_boot_fused<T>(B, lgN, ntaps, 1, s);
+}
+
 template <typename T>
 void Plan<T>::bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx<T> *out_w,
                         hipStream_t s) {
@@ -306,45 +370,14 @@ void Plan<T>::bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx
     const int nc = ncurves();
     prepare_bootstrap(nbs, out_w != nullptr);
     if (bootstrap_fused_ok(out_w != nullptr)) {
-        int *cov = cover.as<int>();
-        if (!cover_ready) {
-            for (int w = 0; w < (has_pts2 ? 2 : 1); ++w) {
-                const TiledPoints<T> &P = w == 0 ? pts : pts2;
-                launch_curve_cover<T>(P.rec_orig.template as<PointRec<T>>(),
-                                      P.coff.template as<int>(), nc, G, ntaps, cov + w * nc, s);
-            }
-            cover_ready = true;
-        }
-        if (y) launch_curve_absmax<T>(y, pts.coff.template as<int>(), nc, ymax.as<T>(), s);
-        BootArgs<T> B{};
-        B.rec_x = pts.rec_orig.template as<PointRec<T>>();
-        B.rec_2x = has_pts2 ? pts2.rec_orig.template as<PointRec<T>>() : nullptr;
-        B.y = y;
-        B.ymax = ymax.as<T>();
-        B.cover_x = cov;
-        B.cover_2x = cov + nc;
-        B.coff = pts.coff.template as<int>();
-        B.order = pts.order.template as<int>();
-        B.counter = counter.as<int>();
+        BootArgs<T> B = fused_args(y, nbs, seed, s);
         B.out_y = out_y;
         B.out_w = out_w;
-        B.ncurves = nc;
-        B.nbs = nbs;
         B.ntr = out_w ? 2 : 1;
         const long total = (long)nc * nbs * B.ntr;
         require(total < (1L << 31), "too many bootstrap items");
         B.total = (int)total;
-        B.seed = seed;
-        B.bi = T(1) / beta;
-        B.wscale = wscale;
-        for (int q = 0; q < kMaxTaps; ++q) B.e3[q] = e3[q];
-        const double pi = 3.14159265358979323846;
-        B.sink.M = M;
-        B.sink.ostride = M;
-        B.sink.invG = T(1) / (T)G;
-        B.sink.coef = (T)(pi * pi * (double)beta);
-        B.sink.pref = (T)(1.0 / std::sqrt(pi * (double)beta));
-        launch_boot_fused<T>(B, lgN, ntaps, s);
+        launch_boot_fused<T>(B, lgN, ntaps, 0, s);
         return;
     }
     const long n_ = n();
@@ -976,6 +1009,15 @@ CUNA_API int cuna_nfft_bootstrap(cuna_nfft_plan plan, const dTyp *y, int nbs, un
         require(plan && plan->p, "plan is NULL");
         plan->p->bootstrap(y, nbs, seed, reinterpret_cast<cplx<R> *>(out_y),
                            reinterpret_cast<cplx<R> *>(out_w), stream);
+    });
+}
+
+CUNA_API int cuna_nfft_bootstrap_ls(cuna_nfft_plan plan, const dTyp *y, int nbs,
+                                    unsigned int seed, dTyp *out_power, dTyp *out_peak,
+                                    int *out_kpeak, hipStream_t stream) {
+    return guarded([&] {
+        require(plan && plan->p, "plan is NULL");
+        plan->p->bootstrap_ls(y, nbs, seed, out_power, out_peak, out_kpeak, stream);
     });
 }
 

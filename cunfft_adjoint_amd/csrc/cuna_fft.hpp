@@ -236,19 +236,25 @@ __device__ __forceinline__ void lds_fft(cplx<T> *lds, const cplx<T> *__restrict_
 template <typename T, int L, int NT>
 struct TableStage {
     static constexpr int K = (L + NT - 1) / NT;
-    cplx<T> v[K];
+    // real and imaginary parts kept as scalars (an array of the complex
+    // struct stayed in scratch memory in the FP64 build)
+    T vx[K], vy[K];
     __device__ __forceinline__ void load(const cplx<T> *__restrict__ g, int tid) {
+        // unconditional (clamped) loads: no branch, so the values stay in
+        // registers and the loads issue together
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = tid + k * NT;
-            if (L % NT == 0 || i < L) v[k] = g[i];
+            const cplx<T> w = g[(L % NT == 0 || i < L) ? i : L - 1];
+            vx[k] = w.x;
+            vy[k] = w.y;
         }
     }
     __device__ __forceinline__ void store(cplx<T> *dst, int tid) const {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = tid + k * NT;
-            if (L % NT == 0 || i < L) dst[i] = v[k];
+            if (L % NT == 0 || i < L) dst[i] = cmk(vx[k], vy[k]);
         }
     }
 };
@@ -260,7 +266,7 @@ constexpr bool lds_table(int lg) { return lg <= 12; }
 // place, the result is in the swizzled layout
 template <typename T, int S, int LG, int NT, class TW>
 __device__ __forceinline__ void lds_fft_inplace(cplx<T> *lds, TW tw) {
-    auto fetch = [&](int e, int) { return lds[e]; };
+    auto fetch = [&](int e, int) __attribute__((always_inline)) { return lds[e]; };
     fft_pass<T, S, LG, 1, NT, 0, 0, decltype(fetch), true, NoHook, TW>(lds, tw, fetch);
 }
 
@@ -475,9 +481,9 @@ __global__ __launch_bounds__(NT) void k_fft_small(Loader ld, Sink sk, cplx<T> *p
             if (r <= N / 2) ot[i].load(twG, r);
         }
     }
-    auto fetch = [&](int e, int) { return lb.at(e); };
+    auto fetch = [&](int e, int) __attribute__((always_inline)) { return lb.at(e); };
     if constexpr (TAB) {
-        auto hook = [&]() { tab.store(twl, tid); };
+        auto hook = [&]() __attribute__((always_inline)) { tab.store(twl, tid); };
         lds_fft<T, S, LG, 1, NT>(lds, twl, fetch, hook);
     } else {
         lds_fft<T, S, LG, 1, NT>(lds, tw, fetch);
@@ -508,7 +514,7 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
     constexpr int L2 = 1 << LG2;
     constexpr bool TAB = Loader::kStage && lds_table(LG2);
     // epilogue twiddles held through the FFT: only where registers are ample
-    constexpr bool PRE = Loader::kStage && NT <= 256;
+    constexpr bool PRE = Loader::kStage && NT <= 256 && sizeof(T) == 4;
     constexpr int OPT = PRE ? (W * L2 + NT - 1) / NT : 1;
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
     cplx<T> *twl = lds + W * L2;
@@ -530,9 +536,9 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
                 ot[i].load(twN, (long)(n10 + idx % W) * (idx / W));
         }
     }
-    auto fetch = [&](int e, int w) { return lb.at(n10 + w + (long)L1 * e); };
+    auto fetch = [&](int e, int w) __attribute__((always_inline)) { return lb.at(n10 + w + (long)L1 * e); };
     if constexpr (TAB) {
-        auto hook = [&]() { tab.store(twl, tid); };
+        auto hook = [&]() __attribute__((always_inline)) { tab.store(twl, tid); };
         lds_fft<T, S, LG2, W, NT>(lds, twl, fetch, hook);
     } else {
         lds_fft<T, S, LG2, W, NT>(lds, tw, fetch);
@@ -596,9 +602,9 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
         if (s < H) return rho;
         return rho == 0 ? L2 / 2 : L2 - rho;
     };
-    auto fetch = [&](int e, int s) { return z[(long)row_of(s) * L1 + e]; };
+    auto fetch = [&](int e, int s) __attribute__((always_inline)) { return z[(long)row_of(s) * L1 + e]; };
     if constexpr (TAB) {
-        auto hook = [&]() { tab.store(twl, tid); };
+        auto hook = [&]() __attribute__((always_inline)) { tab.store(twl, tid); };
         lds_fft<T, 1, LG1, W, NT>(lds, twl, fetch, hook);
     } else {
         lds_fft<T, 1, LG1, W, NT>(lds, tw, fetch);
@@ -652,9 +658,9 @@ __global__ __launch_bounds__(NT) void k_fft_row_plain(const cplx<T> *src, long s
     const cplx<T> *z = src + b * src_stride;
     TableStage<T, (TAB ? L1 : 1), NT> tab;
     if constexpr (TAB) tab.load(tw, tid);
-    auto fetch = [&](int e, int h) { return z[(long)(i * H + h) * L1 + e]; };
+    auto fetch = [&](int e, int h) __attribute__((always_inline)) { return z[(long)(i * H + h) * L1 + e]; };
     if constexpr (TAB) {
-        auto hook = [&]() { tab.store(twl, tid); };
+        auto hook = [&]() __attribute__((always_inline)) { tab.store(twl, tid); };
         lds_fft<T, S, LG1, H, NT>(lds, twl, fetch, hook);
     } else {
         lds_fft<T, S, LG1, H, NT>(lds, tw, fetch);

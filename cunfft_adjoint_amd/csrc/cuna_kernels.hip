@@ -445,15 +445,18 @@ bool boot_fused_supported(int lgN, int ntaps) {
                                                     ntaps == 24 || ntaps == 26 || ntaps == 32);
 }
 
-template <typename T, int LG, int NTAPS>
+template <typename T, int LG, int NTAPS, int MODE>
 static void boot_fused_one(BootArgs<T> A, hipStream_t s) {
-    constexpr int NT = fft_threads(LG, 1) < 256 ? 256 : fft_threads(LG, 1);
+    // the Lomb-Scargle mode holds the window's values across the second
+    // transform: at most 512 threads (256 registers each)
+    constexpr int NT0 = fft_threads(LG, 1) < 256 ? 256 : fft_threads(LG, 1);
+    constexpr int NT = MODE == 1 && NT0 > 512 ? 512 : NT0;
     auto &tc = TwiddleCache<T>::get();
     A.twN = tc.two_level(LG);
     A.twG = tc.two_level(LG + 1);
     const size_t lds = boot_lds_bytes<T>(LG);
     static_assert(boot_lds_bytes<T>(LG) <= kMaxLds, "fused bootstrap LDS");
-    auto k = k_boot_fused<T, LG, NTAPS, NT>;
+    auto k = k_boot_fused<T, LG, NTAPS, NT, MODE>;
     set_lds(k, lds);
     int dev = 0, ncu = 0;
     CUNA_HIP(hipGetDevice(&dev));
@@ -488,19 +491,26 @@ void launch_curve_absmax(const T *y, const int *coff, int ncurves, T *out, hipSt
 }
 
 template <typename T>
-void launch_boot_fused(BootArgs<T> A, int lgN, int ntaps, hipStream_t s) {
+void launch_boot_fused(BootArgs<T> A, int lgN, int ntaps, int mode, hipStream_t s) {
     require(boot_fused_supported<T>(lgN, ntaps), "no fused bootstrap kernel for this size",
             CUNA_ERROR_UNSUPPORTED);
     dispatch_lg<6, max_small_lg<T>()>(lgN, [&](auto c) {
         constexpr int LG = decltype(c)::value;
-        switch (ntaps) {
-            case 12: boot_fused_one<T, LG, 12>(A, s); break;
-            case 16: boot_fused_one<T, LG, 16>(A, s); break;
-            case 20: boot_fused_one<T, LG, 20>(A, s); break;
-            case 24: boot_fused_one<T, LG, 24>(A, s); break;
-            case 26: boot_fused_one<T, LG, 26>(A, s); break;
-            default: boot_fused_one<T, LG, 32>(A, s); break;
-        }
+        auto go = [&](auto m) {
+            constexpr int MODE = decltype(m)::value;
+            switch (ntaps) {
+                case 12: boot_fused_one<T, LG, 12, MODE>(A, s); break;
+                case 16: boot_fused_one<T, LG, 16, MODE>(A, s); break;
+                case 20: boot_fused_one<T, LG, 20, MODE>(A, s); break;
+                case 24: boot_fused_one<T, LG, 24, MODE>(A, s); break;
+                case 26: boot_fused_one<T, LG, 26, MODE>(A, s); break;
+                default: boot_fused_one<T, LG, 32, MODE>(A, s); break;
+            }
+        };
+        if (mode == 0)
+            go(std::integral_constant<int, 0>{});
+        else
+            go(std::integral_constant<int, 1>{});
     });
 }
 
@@ -520,7 +530,7 @@ template void launch_tile_keys<dTyp>(const PointRec<dTyp> *, long, int, int, int
 template void launch_spread<dTyp>(const SpreadArgs<dTyp> &, int, int, int, hipStream_t);
 template void launch_gather<dTyp>(const GatherArgs<dTyp> &, int, int, hipStream_t);
 template bool boot_fused_supported<dTyp>(int, int);
-template void launch_boot_fused<dTyp>(BootArgs<dTyp>, int, int, hipStream_t);
+template void launch_boot_fused<dTyp>(BootArgs<dTyp>, int, int, int, hipStream_t);
 template void launch_curve_cover<dTyp>(const PointRec<dTyp> *, const int *, int, int, int, int *,
                                        hipStream_t);
 template void launch_curve_absmax<dTyp>(const dTyp *, const int *, int, dTyp *, hipStream_t);

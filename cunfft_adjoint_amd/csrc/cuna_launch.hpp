@@ -84,7 +84,8 @@ void launch_curve_cover(const PointRec<T> *rec, const int *coff, int ncurves, in
 template <typename T>
 void launch_curve_absmax(const T *y, const int *coff, int ncurves, T *out, hipStream_t s);
 template <typename T>
-void launch_boot_fused(BootArgs<T> A, int lgN, int ntaps, hipStream_t s);
+// mode 0: spectra of y at x and of the window at 2x; 1: Lomb-Scargle power
+void launch_boot_fused(BootArgs<T> A, int lgN, int ntaps, int mode, hipStream_t s);
 
 // bytes of dynamic LDS a launch may use on gfx950
 constexpr size_t kMaxLds = 160 * 1024;

@@ -100,6 +100,9 @@ struct Plan {
     // accurate bootstrap (resampled points)
     void bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx<T> *out_w,
                    hipStream_t s);
+    // Lomb-Scargle power of each replicate (fused kernel sizes only)
+    void bootstrap_ls(const T *y, int nbs, uint32_t seed, T *power, T *peak, int *kpeak,
+                      hipStream_t s);
     void backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s);
     // allocate every workspace a later call with these sizes needs (so that
     // the call can be captured into a graph without allocating)
@@ -107,6 +110,7 @@ struct Plan {
     bool bootstrap_fused_ok(bool window) const;
 
     SpreadArgs<T> spread_args(const TiledPoints<T> &P) const;
+    BootArgs<T> fused_args(const T *y, int nbs, uint32_t seed, hipStream_t s);
     void run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s,
                  hipEvent_t mid = nullptr) const;
 };

@@ -745,6 +745,11 @@ struct BootArgs {
     SinkAccurate<T> sink;       // out pointer set per item
     Tw2<T> twN;                 // w_N^e, e < N (two-level, staged into LDS)
     Tw2<T> twG;                 // w_G^e, e < G (two-level, staged into LDS)
+    // Lomb-Scargle mode (each nullable): per replicate, power at k = 0..M/2-1,
+    // and its maximum over k >= 1 with the k where it occurs
+    T *ls_power;
+    T *ls_peak;
+    int *ls_kpeak;
 };
 
 // LDS of the fused kernel: the grid (G reals = N complex) and the four
@@ -778,7 +783,41 @@ struct FixedOf<double> {
     __device__ static I round(double v) { return __double2ll_rn(v); }
 };
 
-template <typename T, int LG, int NTAPS, int NT>
+// Spectrum value A[r] of the real grid from the pair (Z[r], Z[N-r]) of its
+// complex transform (the first of the two values emit_pair produces).
+template <typename T>
+__device__ __forceinline__ cplx<T> real_spectrum_at(cplx<T> Zr, cplx<T> Znr, cplx<T> wr) {
+    cplx<T> c = conjg(Znr);
+    cplx<T> E = (Zr + c) * T(0.5);
+    cplx<T> D = (Zr - c) * T(0.5);
+    return E + wr * cmk(D.y, -D.x);
+}
+
+// Lomb-Scargle power (Press & Rybicki 1989, "standard" normalisation, data
+// centred) from F = sum (y_i - ybar) e^{i w t_i} and W = sum e^{2 i w t_i}
+// over n points with yy = sum (y_i - ybar)^2:
+//   tan 2wtau = Im W / Re W,  YC = Re F cos wtau + Im F sin wtau,
+//   YS = Im F cos wtau - Re F sin wtau,  CC = (n + |W|)/2,  SS = (n - |W|)/2,
+//   P = (YC^2/CC + YS^2/SS) / yy   (a term whose denominator vanishes is 0).
+template <typename T>
+__device__ __forceinline__ T ls_power(cplx<T> F, cplx<T> W, T n, T yy) {
+    const T h = sqrt(W.x * W.x + W.y * W.y);
+    const T c2 = h > T(0) ? W.x / h : T(1), s2 = h > T(0) ? W.y / h : T(0);
+    const T cw = sqrt(fmax(T(0), (T(1) + c2) * T(0.5)));
+    const T sw = copysign(sqrt(fmax(T(0), (T(1) - c2) * T(0.5))), s2);
+    const T yc = F.x * cw + F.y * sw, ys = F.y * cw - F.x * sw;
+    const T cc = (n + h) * T(0.5), ss = (n - h) * T(0.5);
+    const T tiny = n * T(sizeof(T) == 4 ? 1e-6 : 1e-14);
+    T p = (cc > tiny ? yc * yc / cc : T(0)) + (ss > tiny ? ys * ys / ss : T(0));
+    return yy > T(0) ? p / yy : T(0);
+}
+
+// MODE 0: items (curve, replicate, transform) -> deconvolved spectra.
+// MODE 1: items (curve, replicate) -> Lomb-Scargle power of the replicate:
+// the window transform (unit weights at 2x) is kept in registers for the
+// frequencies this thread emits, then the transform of the mean-subtracted
+// values at x gives the power directly; nothing else is written.
+template <typename T, int LG, int NTAPS, int NT, int MODE>
 __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
     using I = typename FixedOf<T>::I;
     using U = std::make_unsigned_t<I>;
@@ -787,6 +826,7 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
     constexpr int LBN = (LG + 1) / 2, LBG = (LG + 2) / 2;  // as TwiddleCache::two_level
     constexpr int NLN = 1 << LBN, NHN = 1 << (LG - LBN), NLG = 1 << LBG, NHG = 1 << (LG + 1 - LBG);
     constexpr int PB = 4;  // resampled points per thread whose loads are in flight together
+    constexpr int OPT = (N / 2 + NT - 1) / NT;  // Lomb-Scargle frequencies per thread
     constexpr int kMaxExp = sizeof(T) == 4 ? 100 : 1000;
     T *grid = reinterpret_cast<T *>(cuna_smem);
     U *gu = reinterpret_cast<U *>(cuna_smem);
@@ -805,29 +845,21 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
     twG.hi = tl + NLN + NHN + NLG;
     twG.lb = LBG;
     __shared__ int item_sh, overflow_sh;
-    for (;;) {
-        if (tid == 0) item_sh = atomicAdd(A.counter, 1);
-        __syncthreads();
-        const int item = item_sh;
-        if (item >= A.total) break;  // uniform over the workgroup
-        const int per_curve = A.nbs * A.ntr;
-        const int ci = A.order[item / per_curve];
-        const int rem = item % per_curve;
-        const int rep = rem / A.ntr, tr = rem % A.ntr;
-        const int c0 = A.coff[ci];
-        const int nc = A.coff[ci + 1] - c0;
-        const PointRec<T> *P = tr == 0 ? A.rec_x : A.rec_2x;
-        const bool usey = tr == 0 && A.y;
-        const double vm = usey ? (double)A.ymax[ci] : 1.0;
-        FastMod fm;
-        fm.init((uint32_t)max(nc, 1));
+    __shared__ double red_sh[2 * (NT / 64)];
+    __shared__ T peak_sh[NT / 64];
+    __shared__ int kpk_sh[NT / 64];
+
+    // Grid of one transform: the draws of replicate `rep` of the curve at
+    // c0 (nc points), positions from P, values y[d] - shift (or 1), |value|
+    // <= vm; spread in fixed point, converted, transformed in place into z.
+    auto transform = [&](int c0, int nc, int rep, const FastMod &fm, const PointRec<T> *P,
+                         const T *y, T shift, double vm, int cover) __attribute__((always_inline)) {
         int sexp = 0;
         if (vm <= 1e300) {  // false for NaN and inf
             // first attempt: 4x the densest window of the original points;
             // second: nc points in one window, which cannot overflow
             for (int attempt = 0; attempt < 2; ++attempt) {
-                const int cnt = attempt == 0 ? 4 * max(tr == 0 ? A.cover_x[ci] : A.cover_2x[ci], 1)
-                                             : max(nc, 1);
+                const int cnt = attempt == 0 ? 4 * max(cover, 1) : max(nc, 1);
                 const double bound = vm * (double)cnt * (double)A.wscale;
                 const int e = bound > 0 ? (int)ceil(log2(bound)) : -kMaxExp;
                 sexp = min(max(FixedOf<T>::headroom_bits - e, -kMaxExp), kMaxExp);
@@ -855,9 +887,9 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
                     }
 #pragma unroll
                     for (int k = 0; k < PB; ++k) pr[k] = P[dk[k]];
-                    if (usey) {
+                    if (y) {
 #pragma unroll
-                        for (int k = 0; k < PB; ++k) vv[k] = A.y[dk[k]];
+                        for (int k = 0; k < PB; ++k) vv[k] = y[dk[k]] - shift;
                     } else {
 #pragma unroll
                         for (int k = 0; k < PB; ++k) vv[k] = T(1);
@@ -891,12 +923,142 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
         }
         __syncthreads();
         lds_fft_inplace<T, 1, LG, NT>(z, twN);
-        SinkAccurate<T> sk = A.sink;
-        cplx<T> *o = (tr == 0 ? A.out_y : A.out_w) + ((long)ci * A.nbs + rep) * sk.M;
-        for (int r = tid; r <= N / 2; r += NT)
-            emit_pair(sk, o, (long)r, (long)N, z[lds_sw(r)], z[lds_sw((N - r) & (N - 1))],
-                      twG(r));
+    };
+    // deconvolved spectrum at frequency f (0 <= f < M/2) from z
+    auto spectrum = [&](int f) __attribute__((always_inline)) -> cplx<T> {
+        const SinkAccurate<T> &sk = A.sink;
+        const cplx<T> a = real_spectrum_at(z[lds_sw(f)], z[lds_sw((N - f) & (N - 1))], twG(f));
+        const T kk = (T)f * sk.invG;
+        return a * (sk.pref * dexp(sk.coef * kk * kk));
+    };
+
+    for (;;) {
+        if (tid == 0) item_sh = atomicAdd(A.counter, 1);
         __syncthreads();
+        const int item = item_sh;
+        if (item >= A.total) break;  // uniform over the workgroup
+        const int per_curve = A.nbs * (MODE == 0 ? A.ntr : 1);
+        const int ci = A.order[item / per_curve];
+        const int rem = item % per_curve;
+        const int c0 = A.coff[ci];
+        const int nc = A.coff[ci + 1] - c0;
+        FastMod fm;
+        fm.init((uint32_t)max(nc, 1));
+        if constexpr (MODE == 0) {
+            const int rep = rem / A.ntr, tr = rem % A.ntr;
+            const bool usey = tr == 0 && A.y;
+            transform(c0, nc, rep, fm, tr == 0 ? A.rec_x : A.rec_2x, usey ? A.y : nullptr, T(0),
+                      usey ? (double)A.ymax[ci] : 1.0, tr == 0 ? A.cover_x[ci] : A.cover_2x[ci]);
+            SinkAccurate<T> sk = A.sink;
+            cplx<T> *o = (tr == 0 ? A.out_y : A.out_w) + ((long)ci * A.nbs + rep) * sk.M;
+            for (int r = tid; r <= N / 2; r += NT)
+                emit_pair(sk, o, (long)r, (long)N, z[lds_sw(r)], z[lds_sw((N - r) & (N - 1))],
+                          twG(r));
+            __syncthreads();
+        } else {
+            const int rep = rem;
+            const long row = (long)ci * A.nbs + rep;
+            const int half = (int)(A.sink.M / 2);
+            // moments of the drawn values about the first draw (shifted sums
+            // keep the variance accurate when |mean| >> spread)
+            double s1 = 0.0, s2 = 0.0;
+            T shift = T(0);
+            if (nc > 0) {
+                shift = A.y[c0 + (int)fm(xorshift32(wang_hash(A.seed + (uint32_t)rep * (uint32_t)nc)))];
+                for (int p0 = 0; p0 < nc; p0 += NT * PB) {
+                    T vv[PB];
+#pragma unroll
+                    for (int k = 0; k < PB; ++k) {
+                        int p = p0 + k * NT + tid;
+                        p = p < nc ? p : 0;
+                        const uint32_t t = (uint32_t)rep * (uint32_t)nc + (uint32_t)p;
+                        vv[k] = A.y[c0 + (int)fm(xorshift32(wang_hash(A.seed + t)))];
+                    }
+#pragma unroll
+                    for (int k = 0; k < PB; ++k) {
+                        if (p0 + k * NT + tid < nc) {
+                            const double d = (double)vv[k] - (double)shift;
+                            s1 += d;
+                            s2 += d * d;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                s1 += __shfl_xor(s1, o);
+                s2 += __shfl_xor(s2, o);
+            }
+            if ((tid & 63) == 0) {
+                red_sh[2 * (tid >> 6)] = s1;
+                red_sh[2 * (tid >> 6) + 1] = s2;
+            }
+            __syncthreads();
+            s1 = 0.0;
+            s2 = 0.0;
+            for (int w = 0; w < NT / 64; ++w) {
+                s1 += red_sh[2 * w];
+                s2 += red_sh[2 * w + 1];
+            }
+            const double nd = (double)max(nc, 1);
+            const double mean_off = s1 / nd;
+            const T ybar = (T)((double)shift + mean_off);
+            const T yy = (T)fmax(0.0, s2 - s1 * mean_off);
+            // window W(2w) for this thread's frequencies
+            transform(c0, nc, rep, fm, A.rec_2x, nullptr, T(0), 1.0, A.cover_2x[ci]);
+            cplx<T> W[OPT];
+#pragma unroll
+            for (int i = 0; i < OPT; ++i) {
+                const int f = tid + i * NT;
+                W[i] = f < half ? spectrum(f) : cmk(T(0), T(0));
+            }
+            __syncthreads();
+            // F(w) of the mean-subtracted values, then the power
+            const double vm = (double)A.ymax[ci] + fabs((double)ybar);
+            transform(c0, nc, rep, fm, A.rec_x, A.y, ybar, vm, A.cover_x[ci]);
+            T best = T(-1);
+            int kbest = 0;
+#pragma unroll
+            for (int i = 0; i < OPT; ++i) {
+                const int f = tid + i * NT;
+                if (f < half) {
+                    const T pw = ls_power(spectrum(f), W[i], (T)nc, yy);
+                    if (A.ls_power) A.ls_power[row * half + f] = pw;
+                    if (f >= 1 && pw > best) {
+                        best = pw;
+                        kbest = f;
+                    }
+                }
+            }
+            if (A.ls_peak || A.ls_kpeak) {
+                // block arg-max (ties: smallest k)
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    const T ob = __shfl_xor(best, o);
+                    const int ok = __shfl_xor(kbest, o);
+                    if (ob > best || (ob == best && ok < kbest)) {
+                        best = ob;
+                        kbest = ok;
+                    }
+                }
+                if ((tid & 63) == 0) {
+                    peak_sh[tid >> 6] = best;
+                    kpk_sh[tid >> 6] = kbest;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    for (int w = 1; w < NT / 64; ++w) {
+                        if (peak_sh[w] > best || (peak_sh[w] == best && kpk_sh[w] < kbest)) {
+                            best = peak_sh[w];
+                            kbest = kpk_sh[w];
+                        }
+                    }
+                    if (A.ls_peak) A.ls_peak[row] = best < T(0) ? T(0) : best;
+                    if (A.ls_kpeak) A.ls_kpeak[row] = kbest;
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 

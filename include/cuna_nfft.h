@@ -93,6 +93,23 @@ int cuna_nfft_adjoint(cuna_nfft_plan plan, const dTyp *y, Complex *out, hipStrea
 int cuna_nfft_bootstrap(cuna_nfft_plan plan, const dTyp *y, int nbs, unsigned int seed,
                         Complex *out_y, Complex *out_w, hipStream_t stream);
 
+/* accurate mode, CUNA_PLAN_WINDOW2 plan: Lomb-Scargle periodograms of the
+ * same replicates (Press & Rybicki 1989; data centred on the replicate mean,
+ * "standard" normalisation, 0 <= P <= 1), computed in the fused bootstrap
+ * kernel without writing the spectra.  Frequency k*df (times plans; k/1 for
+ * x plans), k = 0 .. M/2-1:
+ *   P_k = [YC^2/CC + YS^2/SS] / sum (y - ybar)^2,  tan 2wtau = S2/C2,
+ * from F = sum (y_d - ybar) e^{i w t_d} and sum e^{2 i w t_d}.
+ * out_power[c*nbs + s][k] (nullable), out_peak[c*nbs + s] = max over
+ * k >= 1 and out_kpeak[c*nbs + s] its k (smallest on ties; both nullable).
+ * Grids of at most 2^15 cells (FP32) / 2^14 (FP64) and m in {6,8,10,12,13,16};
+ * otherwise CUNA_ERROR_UNSUPPORTED.  No counterpart in the reference (its
+ * callers compute the periodogram from the two spectra of
+ * cunfft_adjoint_raw_async_bootstrap, src/cuna_kernel.cu:236-289). */
+int cuna_nfft_bootstrap_ls(cuna_nfft_plan plan, const dTyp *y, int nbs, unsigned int seed,
+                           dTyp *out_power, dTyp *out_peak, int *out_kpeak,
+                           hipStream_t stream);
+
 /* accurate mode: given grad_out = dL/dRe F + i dL/dIm F (ncurves x M complex),
  * writes dL/dy (n) and dL/dx (n); either output may be NULL.  For a times
  * plan dL/dt_j = df * dL/dx_j. */

@@ -65,6 +65,7 @@ def lib():
         L.orc_ndft_bands.argtypes = [_d, _d, _int, _i64, _int, _int, _d]
         L.orc_ndft_grad.argtypes = [_d, _d, _int, _int, _d, _d, _d]
         L.orc_num_threads.restype = _int
+        L.orc_lomb_scargle.argtypes = [_d, _d, _int, ctypes.c_double, ctypes.c_double, _int, _d]
         _lib = L
     return _lib
 
@@ -200,3 +201,13 @@ def ndft_grad(x, y, M: int, g):
     lib().orc_ndft_grad(_dp(x), _dp(y), n, M, g.view(np.float64).ctypes.data_as(_d), _dp(dy),
                         _dp(dx))
     return dy, dx
+
+
+def lomb_scargle(t, y, df: float, nf: int, f0: float = 0.0) -> np.ndarray:
+    """Lomb-Scargle power (centred data, standard normalisation) at
+    f0 + k df, k = 0 .. nf-1, by the direct FP64 sums."""
+    t = _f64(t)
+    y = _f64(y)
+    out = np.empty(nf, dtype=np.float64)
+    lib().orc_lomb_scargle(_dp(t), _dp(y), len(t), float(f0), float(df), int(nf), _dp(out))
+    return out

@@ -59,6 +59,10 @@ void orc_ndft_grad(const double *x, const double *y, int n, int M, const double 
 /* number of OpenMP threads the oracle uses */
 int orc_num_threads(void);
 
+/* Lomb-Scargle power at f0 + k df, k < nf (direct FP64) */
+void orc_lomb_scargle(const double *t, const double *y, int n, double f0, double df, int nf,
+                      double *power);
+
 #ifdef __cplusplus
 }
 #endif

@@ -380,6 +380,47 @@ def test_fused_batched_bootstrap_vs_unfused_and_oracle():
     assert rel_l2(oy, fy[c * nbs:(c + 1) * nbs]) < 1e-6
 
 
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_bootstrap_lomb_scargle_vs_oracle(prec):
+    """Fused Lomb-Scargle of bootstrap replicates vs the direct FP64
+    periodogram of the same draws; the peak is the row's arg-max over k >= 1."""
+    from tests._data import light_curves
+    t, y, offs = light_curves(6, 21, nmin=50, nmax=2000)
+    df, M, nbs, seed = 1.0 / 3650.0, 1 << 12, 3, 17
+    yv = y.astype(NP[prec])
+    plan = cu.NFFTPlan(t=dev(t), df=df, offsets=offs, M=M, window2=True,
+                       dtype=torch.float32 if prec == "f" else torch.float64)
+    r = plan.bootstrap_ls(dev(yv), nbs, seed)
+    pw, pk, kp = host(r["power"]), host(r["peak"]), host(r["kpeak"])
+    assert pw.shape == (6 * nbs, M // 2)
+    tol = 2e-5 if prec == "f" else 1e-10
+    worst = 0.0
+    for c in range(6):
+        n = offs[c + 1] - offs[c]
+        d = oracle.bootstrap_indices(seed, n, nbs) + offs[c]
+        for s in range(nbs):
+            ref = oracle.lomb_scargle(t[d[s]], yv.astype(np.float64)[d[s]], df, M // 2)
+            got = pw[c * nbs + s]
+            worst = max(worst, float(np.max(np.abs(got - ref))))
+            k = 1 + int(np.argmax(got[1:]))
+            assert kp[c * nbs + s] == k and pk[c * nbs + s] == got[k]
+    assert worst < tol, worst
+    # peak-only call: same draws, same peaks
+    r2 = plan.bootstrap_ls(dev(yv), nbs, seed, power=False)
+    assert r2["power"] is None
+    assert np.array_equal(host(r2["peak"]), pk) and np.array_equal(host(r2["kpeak"]), kp)
+
+
+def test_bootstrap_lomb_scargle_rejects_unsupported():
+    x, y = uniform_points(1000, 3)
+    plan = cu.NFFTPlan(dev(x), 1 << 10)  # no window2
+    with pytest.raises(cu.CunaError):
+        plan.bootstrap_ls(dev(y), 2, 1)
+    big = cu.NFFTPlan(dev(x), 1 << 16, window2=True)  # grid 2^17: no fused kernel
+    with pytest.raises(cu.CunaError):
+        big.bootstrap_ls(dev(y), 2, 1)
+
+
 def test_graph_replays_match_eager():
     stream = torch.cuda.Stream()
     x, y = clustered_points(100_000, 9)

@@ -266,3 +266,32 @@ def test_compat_params_closed_form():
         bf, nff, rf = oracle.compat_params(n, ng, False)
         assert rf == 6 and bf == pytest.approx(2 * R / (2 * R - 1) * 6 / np.pi, rel=1e-6)
         assert np.float32(bf) == bf
+
+
+# ------------------------------------------------------- Lomb-Scargle --
+def test_lomb_scargle_equals_least_squares_sinusoid_fit():
+    """The classical periodogram of centred data is the fraction of variance
+    explained by the least-squares fit of a cos wt + b sin wt (the tau shift
+    only rotates the basis); numpy lstsq gives that independently."""
+    rng = np.random.default_rng(70)
+    t = np.sort(rng.uniform(0, 100.0, 300))
+    y = np.sin(2 * np.pi * 0.137 * t + 0.4) + 0.5 * rng.standard_normal(300)
+    df, nf = 0.01, 60
+    P = oracle.lomb_scargle(t, y, df, nf)
+    yc = y - y.mean()
+    for k in range(1, nf):
+        w = 2 * np.pi * k * df
+        X = np.stack([np.cos(w * t), np.sin(w * t)], axis=1)
+        coef, *_ = np.linalg.lstsq(X, yc, rcond=None)
+        rss = np.sum((yc - X @ coef) ** 2)
+        assert abs(P[k] - (1 - rss / np.sum(yc ** 2))) < 1e-10, k
+    assert np.all(P >= 0) and np.all(P <= 1 + 1e-12)
+
+
+def test_lomb_scargle_peak_and_edge_cases():
+    t = np.linspace(0, 50, 400, endpoint=False)
+    y = np.cos(2 * np.pi * 0.5 * t)
+    P = oracle.lomb_scargle(t, y, 0.02, 50)
+    assert int(np.argmax(P)) == 25 and abs(P[25] - 1) < 1e-9
+    # constant data: no variance, power 0
+    assert np.all(oracle.lomb_scargle(t, np.full(400, 3.0), 0.02, 10) == 0)
