@@ -244,7 +244,7 @@ def bench_c2(args, torch, cu):
     G = info.G
     bytes_total = N * (4 + 4) + 2 * G * 4 + M * 8
     kernels = {
-        "spread (k_spread)": (spread, N * (4 + 4) + G * 4, "k_spread"),
+        "spread (k_spread_fx)": (spread, N * (4 + 4) + G * 4, "k_spread_fx"),
         "fft pass 1 (k_fft_col)": (colp, G * 4 + (G // 2) * 8, "k_fft_col"),
         "fft pass 2 + deconvolution (k_fft_row_real)": (rowp, (G // 2) * 8 + M * 8,
                                                          "k_fft_row_real"),
