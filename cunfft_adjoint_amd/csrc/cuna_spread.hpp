@@ -383,7 +383,36 @@ __device__ __forceinline__ void tap_weights(T v, T e, T bi, const T *e3, T *w) {
     }
 }
 
-// value and weight parameter of sorted point s (original index p)
+// The same weights by two ratio recurrences, which need exp(-bi) instead of
+// the whole E3 table (16 fewer scalar registers): with w_o = v
+// exp(-(o-e)^2 bi), w_{o+1} = w_o R_o with R_0 = E2 exp(-bi),
+// R_{o+1} = R_o exp(-2 bi); downwards w_{o-1} = w_o L_o with
+// L_0 = exp(-bi)/E2, L_{o-1} = L_o exp(-2 bi).
+template <typename T, int NTAPS>
+__device__ __forceinline__ void tap_weights_rec(T v, T e, T bi, T c1, T *w) {
+    constexpr int Q0 = (NTAPS - 1) / 2;
+    const T D = c1 * c1;
+    const T E1 = dexp(-e * e * bi);
+    const T E2 = dexp(T(2) * e * bi);
+    const T E2i = dexp(T(-2) * e * bi);
+    const T w0 = v * E1;
+    T up = w0, r = E2 * c1;
+#pragma unroll
+    for (int q = Q0; q < NTAPS; ++q) {
+        w[q] = up;
+        up *= r;
+        r *= D;
+    }
+    T dn = w0, l = E2i * c1;
+#pragma unroll
+    for (int q = Q0 - 1; q >= 0; --q) {
+        dn *= l;
+        l *= D;
+        w[q] = dn;
+    }
+}
+
+// value and weight parameter of sorted point s (original index p)// value and weight parameter of sorted point s (original index p)
 template <typename T, int VM>
 __device__ __forceinline__ void point_value(const SpreadArgs<T> &A, int p, int c, int rep, T &v,
                                             T &e) {
@@ -610,6 +639,7 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
     const int total = R.total;
     const bool multi = total > CAP;
     const int tid = threadIdx.x;
+    const float c1 = A.e3[Q0 + 1];  // exp(-bi): e3[q] = exp(-(q - Q0)^2 bi)
 
     for (int g = tid; g < NC; g += NT) acc[g] = 0ull;
     if (multi)
@@ -670,7 +700,7 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
             for (int k = 0; k < PPT; ++k) {
                 if (base + k * NT + tid < base + cnt) {
                     float w[NTAPS];
-                    tap_weights<float, NTAPS>(vv[k] * mul, ev[k], A.bi, A.e3, w);
+                    tap_weights_rec<float, NTAPS>(vv[k] * mul, ev[k], A.bi, c1, w);
                     const int c0 = am[k] - Q0;  // first tap's cell
                     if (R.tiny) {
 #pragma unroll
