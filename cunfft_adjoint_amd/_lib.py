@@ -108,6 +108,7 @@ def _protos(prec: str):
         "cuna_nfft_adjoint": (c_int, [P, P, P, P]),
         "cuna_nfft_bootstrap": (c_int, [P, P, c_int, c_uint, P, P, P]),
         "cuna_nfft_bootstrap_ls": (c_int, [P, P, c_int, c_uint, P, P, P, P]),
+        "cuna_nfft_forward": (c_int, [P, P, P, P]),
         "cuna_nfft_backward": (c_int, [P, P, P, P, P, P]),
         "cuna_nfft_plan_info": (c_int, [P, P]),
         "cuna_nfft_set_profile_events": (c_int, [P, P, c_int]),

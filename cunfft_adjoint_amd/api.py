@@ -326,6 +326,17 @@ class NFFTPlan:
         return {"power": out_power if power else None, "peak": out_peak if peak else None,
                 "kpeak": out_kpeak if peak else None}
 
+    def forward(self, fhat, out=None, stream=None):
+        """Forward (type-2) NFFT: f_j = sum_k fhat_k e^{-2 pi i k x_j} with fhat
+        of shape (ncurves, M) in the adjoint's layout; returns (n,) complex."""
+        import torch
+
+        if out is None:
+            out = torch.empty(self.n, dtype=complex_dtype(self.prec), device="cuda")
+        fh = fhat.contiguous()
+        check(self.L, self.L.cuna_nfft_forward(self.h, _ptr(fh), _ptr(out), _stream(stream)))
+        return out
+
     def backward(self, y, grad_out, want_y: bool = True, want_x: bool = True, stream=None):
         import torch
 

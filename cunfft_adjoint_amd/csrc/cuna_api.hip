@@ -417,6 +417,7 @@ void Plan<T>::backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s
     work.ensure(sizeof(cplx<T>) * (size_t)N * nc);
     const double pi = 3.14159265358979323846;
     LoadHermitian<T> ld;
+    ld.rot = 0;
     ld.g = g;
     ld.gstride = M;
     ld.M = M;
@@ -442,6 +443,7 @@ void Plan<T>::backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s
     A.dy = dy;
     A.dx = dx;
     A.out_stride = 0;
+    A.dy_elem = 1;
     A.G = G;
     A.tlog = pts.tlog;
     A.ntiles = pts.ntiles;
@@ -453,6 +455,57 @@ void Plan<T>::backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s
     A.dxscale = (T)(2.0 * G / (double)beta);
     for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
     launch_gather<T>(A, ntaps, nc, s);
+}
+
+// Forward (type-2) NFFT f_j = sum_k fhat_k e^{-2 pi i k x_j}: the real part is
+// what backward() computes for dL/dy, Re sum_k fhat_k e^{-2 pi i k x_j}, and
+// by linearity the imaginary part is the same with -i fhat; each pass writes
+// its part of the interleaved complex output.
+template <typename T>
+void Plan<T>::forward(const cplx<T> *fhat, cplx<T> *f, hipStream_t s) {
+    require(spec.mode == CUNA_MODE_ACCURATE, "forward needs an accurate-mode plan");
+    require(fhat != nullptr && f != nullptr, "NULL pointer");
+    const int nc = ncurves();
+    const long N = (long)G / 2;
+    grid.ensure(sizeof(T) * (size_t)G * nc);
+    work.ensure(sizeof(cplx<T>) * (size_t)N * nc);
+    const double pi = 3.14159265358979323846;
+    for (int part = 0; part < 2; ++part) {
+        LoadHermitian<T> ld;
+        ld.rot = part;
+        ld.g = fhat;
+        ld.gstride = M;
+        ld.M = M;
+        ld.G = G;
+        ld.N = N;
+        ld.invG = T(1) / (T)G;
+        ld.coef = (T)(pi * pi * (double)beta);
+        ld.pref = (T)(1.0 / std::sqrt(pi * (double)beta));
+        ld.twG = TwiddleCache<T>::get().two_level(lgN + 1);
+        FftShape sh;
+        sh.lgN = lgN;
+        sh.batch = nc;
+        sh.w = spec.fft_w;
+        fft_c2r_inverse<T>(sh, ld, work.as<cplx<T>>(), grid.as<T>(), s);
+        GatherArgs<T> A{};
+        A.rec = pts.rec.template as<PointRec<T>>();
+        A.perm = pts.perm.template as<int>();
+        A.bucket = pts.bucket.template as<int>();
+        A.h = grid.as<T>();
+        A.h_stride = G;
+        A.dy = reinterpret_cast<T *>(f) + part;
+        A.dy_elem = 2;
+        A.G = G;
+        A.tlog = pts.tlog;
+        A.ntiles = pts.ntiles;
+        A.Kc = pts.Kc;
+        A.olo = pts.olo;
+        A.nrep = 1;
+        A.bi = T(1) / beta;
+        A.wscale = wscale;
+        for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
+        launch_gather<T>(A, ntaps, nc, s);
+    }
 }
 
 // 512-cell tiles: many small workgroups keep the whole chip busy and bound
@@ -1018,6 +1071,15 @@ CUNA_API int cuna_nfft_bootstrap_ls(cuna_nfft_plan plan, const dTyp *y, int nbs,
     return guarded([&] {
         require(plan && plan->p, "plan is NULL");
         plan->p->bootstrap_ls(y, nbs, seed, out_power, out_peak, out_kpeak, stream);
+    });
+}
+
+CUNA_API int cuna_nfft_forward(cuna_nfft_plan plan, const Complex *fhat, Complex *f,
+                               hipStream_t stream) {
+    return guarded([&] {
+        require(plan && plan->p, "plan is NULL");
+        plan->p->forward(reinterpret_cast<const cplx<R> *>(fhat), reinterpret_cast<cplx<R> *>(f),
+                         stream);
     });
 }
 

@@ -408,6 +408,7 @@ template <typename T>
 struct LoadHermitian {
     static constexpr bool kStage = false;  // heavy loader: twiddles stay in global memory
     const cplx<T> *g;  // M values, natural order k = -M/2..M/2-1
+    int rot;           // 1: use -i g (the grid then gives the imaginary part)
     long gstride;      // per batch
     long M, G, N;
     T invG, coef, pref;
@@ -423,7 +424,8 @@ struct LoadHermitian {
             return cmk(T(0), T(0));
         T kk = (T)k * invG;
         T D = pref * dexp(coef * kk * kk);
-        return g[M / 2 + k] * D;
+        const cplx<T> v = g[M / 2 + k] * D;
+        return rot ? cmk(v.y, -v.x) : v;  // -i v: imaginary part of the forward transform
     }
     __device__ __forceinline__ cplx<T> H(long f) const {
         long mf = f == 0 ? 0 : G - f;

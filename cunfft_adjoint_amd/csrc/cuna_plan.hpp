@@ -104,6 +104,8 @@ struct Plan {
     void bootstrap_ls(const T *y, int nbs, uint32_t seed, T *power, T *peak, int *kpeak,
                       hipStream_t s);
     void backward(const T *y, const cplx<T> *g, T *dy, T *dx, hipStream_t s);
+    // forward (type-2) NFFT: fhat (ncurves x M) -> f (n complex)
+    void forward(const cplx<T> *fhat, cplx<T> *f, hipStream_t s);
     // allocate every workspace a later call with these sizes needs (so that
     // the call can be captured into a graph without allocating)
     void prepare_bootstrap(int nbs, bool window);

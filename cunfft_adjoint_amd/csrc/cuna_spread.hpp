@@ -1151,6 +1151,7 @@ struct GatherArgs {
     T *dy;              // nullable
     T *dx;              // nullable
     long out_stride;    // per rep
+    int dy_elem;        // element stride of dy (2: one part of an interleaved complex array)
     int G, tlog, ntiles, Kc, olo, nrep;
     T bi, wscale, dxscale;  // dxscale = 2 G / beta
     T e3[kMaxTaps];
@@ -1191,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs<T> A) {
             sd += ((T)(q - Q0) - e) * hv;
         }
         const long oo = (long)rep * A.out_stride;
-        if (A.dy) A.dy[oo + p] = sw;
+        if (A.dy) A.dy[oo + (long)p * A.dy_elem] = sw;
         if (A.dx) {
             const T v = A.y ? A.y[(long)rep * A.y_stride + p] : T(1);
             A.dx[oo + p] = A.dxscale * v * sd;

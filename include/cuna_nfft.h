@@ -110,6 +110,13 @@ int cuna_nfft_bootstrap_ls(cuna_nfft_plan plan, const dTyp *y, int nbs, unsigned
                            dTyp *out_power, dTyp *out_peak, int *out_kpeak,
                            hipStream_t stream);
 
+/* accurate mode: forward (type-2) NFFT, the adjoint of cuna_nfft_adjoint:
+ * f[j] = sum_k fhat[c][M/2 + k] e^{-2 pi i k x_j}, k = -M/2 .. M/2-1, for the
+ * points j of curve c (x_j = frac(df t_j) for times plans).  fhat: ncurves x M
+ * complex (the adjoint's output layout); f: n complex.  No counterpart in the
+ * reference, which has only the adjoint transform. */
+int cuna_nfft_forward(cuna_nfft_plan plan, const Complex *fhat, Complex *f, hipStream_t stream);
+
 /* accurate mode: given grad_out = dL/dRe F + i dL/dIm F (ncurves x M complex),
  * writes dL/dy (n) and dL/dx (n); either output may be NULL.  For a times
  * plan dL/dt_j = df * dL/dx_j. */

@@ -66,6 +66,7 @@ def lib():
         L.orc_ndft_grad.argtypes = [_d, _d, _int, _int, _d, _d, _d]
         L.orc_num_threads.restype = _int
         L.orc_lomb_scargle.argtypes = [_d, _d, _int, ctypes.c_double, ctypes.c_double, _int, _d]
+        L.orc_ndft_forward.argtypes = [_d, _int, _d, _int, _d]
         _lib = L
     return _lib
 
@@ -211,3 +212,13 @@ def lomb_scargle(t, y, df: float, nf: int, f0: float = 0.0) -> np.ndarray:
     out = np.empty(nf, dtype=np.float64)
     lib().orc_lomb_scargle(_dp(t), _dp(y), len(t), float(f0), float(df), int(nf), _dp(out))
     return out
+
+
+def ndft_forward(x, fhat) -> np.ndarray:
+    """f_j = sum_k fhat_k exp(-2 pi i k x_j), k = -M/2 .. M/2-1 (fhat natural order)."""
+    x = _f64(x)
+    fh = np.ascontiguousarray(fhat, dtype=np.complex128)
+    out = np.empty(2 * len(x), dtype=np.float64)
+    lib().orc_ndft_forward(_dp(x), len(x), fh.view(np.float64).ctypes.data_as(_d), len(fh),
+                           _dp(out))
+    return out.view(np.complex128)

@@ -566,3 +566,33 @@ void orc_lomb_scargle(const double *t, const double *y, int n, double f0, double
         power[k] = yy > 0.0 ? p / yy : 0.0;
     }
 }
+
+/* Forward (type-2) NDFT, the adjoint of orc_ndft_bands over k = -M/2..M/2-1:
+ * f_j = sum_k fhat_k exp(-2 pi i k x_j) (fhat and f interleaved complex),
+ * with the same exact phase reduction, re-seeded every ORC_RESEED modes.
+ * No reference counterpart (CUNA has only the adjoint transform). */
+void orc_ndft_forward(const double *x, int n, const double *fhat, int M, double *f) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int j = 0; j < n; j++) {
+        double sr = 0.0, si = 0.0;
+        for (int c0 = 0; c0 < M; c0 += ORC_RESEED) {
+            int cnt = M - c0 < ORC_RESEED ? M - c0 : ORC_RESEED;
+            double zr, zi, wr, wi;
+            /* exp(+2 pi i k x) then conjugated */
+            phase_root((int64_t)c0 - M / 2, x[j], &zr, &zi);
+            phase_root(1, x[j], &wr, &wi);
+            for (int q = 0; q < cnt; q++) {
+                int i = c0 + q;
+                double gr = fhat[2 * i], gi = fhat[2 * i + 1];
+                /* g * conj(z) */
+                sr += gr * zr + gi * zi;
+                si += gi * zr - gr * zi;
+                double tr = zr * wr - zi * wi;
+                zi = zr * wi + zi * wr;
+                zr = tr;
+            }
+        }
+        f[2 * j] = sr;
+        f[2 * j + 1] = si;
+    }
+}

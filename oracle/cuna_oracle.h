@@ -59,6 +59,9 @@ void orc_ndft_grad(const double *x, const double *y, int n, int M, const double 
 /* number of OpenMP threads the oracle uses */
 int orc_num_threads(void);
 
+/* forward (type-2) NDFT over k = -M/2..M/2-1 (interleaved complex) */
+void orc_ndft_forward(const double *x, int n, const double *fhat, int M, double *f);
+
 /* Lomb-Scargle power at f0 + k df, k < nf (direct FP64) */
 void orc_lomb_scargle(const double *t, const double *y, int n, double f0, double df, int nf,
                       double *power);

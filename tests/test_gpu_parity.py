@@ -331,6 +331,40 @@ def test_backward_vs_oracle(prec, m):
     assert rel_l2(dx, adx) < tol
 
 
+@pytest.mark.parametrize("prec,m", [("f", 8), ("d", 13)])
+def test_forward_nfft_vs_oracle_and_adjointness(prec, m):
+    n, M = 3000, 1 << 10
+    x, y = uniform_points(n, 33, NP[prec])
+    rng = np.random.default_rng(34)
+    g = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    plan = cu.NFFTPlan(dev(x), M, m=m)
+    gd = dev(g.astype(np.complex64 if prec == "f" else np.complex128)).reshape(1, M)
+    f = host(plan.forward(gd))
+    ref = oracle.ndft_forward(x.astype(np.float64), g)
+    tol = 1e-5 if prec == "f" else 1e-11
+    assert rel_l2(f, ref) < tol
+    # adjoint pair on the device: <forward g, y> = <g, adjoint y>
+    F = host(plan.adjoint(dev(y)))[0]
+    lhs = np.vdot(f.astype(np.complex128), y.astype(np.float64))
+    rhs = np.vdot(g, F.astype(np.complex128))
+    assert abs(lhs - rhs) < (1e-4 if prec == "f" else 1e-10) * abs(rhs)
+
+
+def test_forward_nfft_batch_times():
+    from tests._data import light_curves
+    t, _, offs = light_curves(5, 8, nmin=20, nmax=1500)
+    df, M = 1.0 / 3650.0, 1 << 9
+    rng = np.random.default_rng(35)
+    g = (rng.standard_normal((5, M)) + 1j * rng.standard_normal((5, M))).astype(np.complex64)
+    plan = cu.NFFTPlan(t=dev(t), df=df, offsets=offs, M=M)
+    f = host(plan.forward(dev(g)))
+    xs = (df * t) % 1.0
+    for c in range(5):
+        sl = slice(offs[c], offs[c + 1])
+        ref = oracle.ndft_forward(xs[sl], g[c].astype(np.complex128))
+        assert rel_l2(f[sl], ref) < 1e-5
+
+
 def test_autograd_batched_loss():
     B, n, M = 4, 2000, 1 << 12
     rng = np.random.default_rng(40)

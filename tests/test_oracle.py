@@ -295,3 +295,20 @@ def test_lomb_scargle_peak_and_edge_cases():
     assert int(np.argmax(P)) == 25 and abs(P[25] - 1) < 1e-9
     # constant data: no variance, power 0
     assert np.all(oracle.lomb_scargle(t, np.full(400, 3.0), 0.02, 10) == 0)
+
+
+# ------------------------------------------------------- forward NDFT --
+def test_ndft_forward_brute_force_and_adjointness():
+    rng = np.random.default_rng(80)
+    x = rng.uniform(-0.5, 0.5, 200)
+    M = 64
+    g = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    k = np.arange(-M // 2, M // 2)
+    brute = np.exp(-2j * np.pi * np.outer(x, k)) @ g
+    f = oracle.ndft_forward(x, g)
+    assert np.max(np.abs(f - brute)) < 1e-11
+    # <F2 g, y> = <g, F y> with F the (independently pinned) adjoint NDFT
+    y = rng.standard_normal(200)
+    lhs = np.vdot(f, y)                      # sum conj(f_j) y_j
+    rhs = np.vdot(g, oracle.ndft(x, y, M))   # sum conj(g_k) F_k
+    assert abs(lhs - rhs) < 1e-9 * abs(rhs)
