@@ -455,6 +455,21 @@ def test_bootstrap_lomb_scargle_rejects_unsupported():
         big.bootstrap_ls(dev(y), 2, 1)
 
 
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_fft_stage_matches_golden_fixtures(prec):
+    """The FFT stage (CUFFT_INVERSE convention, unnormalised) against the
+    committed numpy fixtures of tests/golden/."""
+    import os
+    gz = np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_inverse_unnormalised.npz"))
+    cdt = torch.complex64 if prec == "f" else torch.complex128
+    for n in gz["sizes"]:
+        d = torch.tensor(gz[f"x{n}"], dtype=cdt, device="cuda")
+        got = host(cu.fft(d, +1)).astype(np.complex128)
+        ref = gz[f"X{n}"]
+        tol = (2e-6 if prec == "f" else 1e-13) * np.log2(max(int(n), 2))
+        assert rel_l2(got, ref) < tol, int(n)
+
+
 def test_graph_replays_match_eager():
     stream = torch.cuda.Stream()
     x, y = clustered_points(100_000, 9)

@@ -8,6 +8,8 @@ float32 step-by-step emulation of the reference's arithmetic (E1 * E2^m * E3
 products, src/gaussian_filter.cu:204-273) written with numpy, against the
 oracle's double-precision closed form exp(-(eps-m)^2/b)/sqrt(pi b).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -312,3 +314,17 @@ def test_ndft_forward_brute_force_and_adjointness():
     lhs = np.vdot(f, y)                      # sum conj(f_j) y_j
     rhs = np.vdot(g, oracle.ndft(x, y, M))   # sum conj(g_k) F_k
     assert abs(lhs - rhs) < 1e-9 * abs(rhs)
+
+
+# ------------------------------------------------- FFT golden fixtures --
+def _fft_golden():
+    path = os.path.join(os.path.dirname(__file__), "golden", "fft_inverse_unnormalised.npz")
+    return np.load(path)
+
+
+def test_fft_matches_golden_fixtures():
+    gz = _fft_golden()
+    for n in gz["sizes"]:
+        got = oracle.fft(gz[f"x{n}"], +1)
+        ref = gz[f"X{n}"]
+        assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref)) * np.log2(max(n, 2)), n
