@@ -98,8 +98,10 @@ static void run_row_plain(const FftShape &sh, const cplx<T> *src, int L2, cplx<T
 }
 
 // Sequences per workgroup of a pass: the largest count whose LDS tile fits in
-// 64 KB that still gives >= 512 workgroups (two or more per CU); an
-// environment override (CUNA_FFT_W, CUNA_FFT_H) exists for tuning runs.
+// 64 KB that still gives >= min_wg workgroups; an environment override
+// (CUNA_FFT_W, CUNA_FFT_H) exists for tuning runs.  Measured at N = 2^20
+// (1024 x 1024): the column pass is fastest at one 256-thread workgroup per
+// CU (W = 4), the row pass with mirror pairs at two or more per CU (H = 1).
 template <typename T, int LG, int W>
 constexpr bool lds_fits() {
     return (long)W * (1L << LG) * (long)sizeof(cplx<T>) <= 65536;
@@ -110,9 +112,24 @@ static int env_int(const char *name) {
     return v ? atoi(v) : 0;
 }
 
+// compute units of the current device (cached per device)
+static int num_cus() {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    int dev = 0;
+    CUNA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    CUNA_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    cache[dev] = std::max(n, 1);
+    return cache[dev];
+}
+
 template <typename T>
 static int pick_seq(int lg, long nseq, int batch, int per_wg_mult, const char *env,
-                    int requested = 0) {
+                    int requested = 0, long min_wg = 512) {
     int forced = requested ? requested : env_int(env);
     auto valid = [&](int w) {
         long lds = (long)w * per_wg_mult * (1L << lg) * (long)sizeof(cplx<T>);
@@ -123,7 +140,7 @@ static int pick_seq(int lg, long nseq, int batch, int per_wg_mult, const char *e
     int best = 1;
     for (int w = 8; w >= 1; w >>= 1) {
         if (!valid(w)) continue;
-        if ((nseq / (w * per_wg_mult)) * (long)batch >= 512) return w;
+        if ((nseq / (w * per_wg_mult)) * (long)batch >= min_wg) return w;
         best = std::min(best, w);
     }
     return best;
@@ -183,7 +200,7 @@ static void real_forward_impl(const FftShape &sh, T *grid, const Sink &sk, hipSt
     }
     const int lg1 = split_lg1(sh.lgN), lg2 = sh.lgN - lg1;
     cplx<T> *z = reinterpret_cast<cplx<T> *>(grid);
-    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W", sh.w);
+    const int W = pick_seq<T>(lg2, 1L << lg1, sh.batch, 1, "CUNA_FFT_W", sh.w, num_cus());
     const int H = std::min(4, pick_seq<T>(lg1, 1L << lg2, sh.batch, 2, "CUNA_FFT_H", sh.h));
     dispatch_lg<7, 12>(lg2, [&](auto c) {
         constexpr int LG2 = decltype(c)::value;
