@@ -719,10 +719,15 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
         }
         __syncthreads();
         if (multi) {
+            // fold the round into facc; each thread clears the cells it read
+            // (a different thread-to-cell map for the clear would race with
+            // the reads of other waves), and the halo cells no thread reads
             const float inv = ldexpf(1.f, -sexp);
-            for (int g = tid; g < TT; g += NT)
+            for (int g = tid; g < TT; g += NT) {
                 facc[g] += (float)(long long)acc[HALO + g] * inv;
-            for (int g = tid; g < NC; g += NT) acc[g] = 0ull;
+                acc[HALO + g] = 0ull;
+            }
+            for (int h = tid; h < 2 * HALO; h += NT) acc[h < HALO ? h : TT + h] = 0ull;
             __syncthreads();
         }
     }
