@@ -3,13 +3,19 @@
 //
 // Replaces fast_gridding / fast_gridding_bootstrap (src/gridding.cu:36-117),
 // which scatter every tap of every point with a global atomicAdd.  Here the
-// points of one x are binned once per plan by a stable counting sort into
-// buckets key = 3*tile + class (class 0: taps reach the previous tile, 2: the
-// next tile, 1: interior).  One workgroup owns one tile: it walks the points
-// of its own bucket plus the boundary classes of its two neighbours,
-// accumulates the taps that land in its tile in LDS (one private copy per
-// wave, summed in a fixed order, so the result is bitwise reproducible) and
-// writes the tile once with coalesced stores.  No global atomics.
+// points of one x are binned once per plan by a stable two-pass counting sort
+// into buckets key = 3*tile + class (class 0: taps reach the previous tile,
+// 2: the next tile, 1: interior), ordered by anchor within a bucket.  One
+// workgroup owns one tile: it takes the points of its own bucket plus the
+// boundary classes of its two neighbours and accumulates the taps that land
+// in its tile in LDS, then writes the tile once with coalesced stores.  No
+// global atomics, and both accumulation forms are bitwise reproducible:
+//   k_spread_fx (FP32 default): scatter with 64-bit integer atomics in fixed
+//     point (integer sums do not depend on their order);
+//   k_spread (FP64, or FP32 on request): gather, each cell summed by one
+//     thread in a fixed order.
+// The fused bootstrap kernel (k_boot_fused) keeps a whole curve's grid in
+// LDS; the backward's gather (k_gather) is the transpose of the spread.
 #pragma once
 
 #include <type_traits>

@@ -445,6 +445,63 @@ def test_bootstrap_lomb_scargle_vs_oracle(prec):
     assert np.array_equal(host(r2["peak"]), pk) and np.array_equal(host(r2["kpeak"]), kp)
 
 
+def test_bootstrap_lomb_scargle_edge_curves_and_graph_replay():
+    """Empty, one-point and constant curves give zero power; replays of a
+    captured call are bitwise equal to the eager call."""
+    rng = np.random.default_rng(61)
+    lens = [0, 1, 300, 50, 700]
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    t = np.concatenate([np.sort(rng.uniform(0, 3650, n)) for n in lens])
+    y = rng.standard_normal(offs[-1]).astype(np.float32)
+    y[offs[3]:offs[4]] = 2.5  # constant curve
+    M, nbs, seed = 1 << 10, 4, 5
+    plan = cu.NFFTPlan(t=dev(t), df=1 / 3650.0, offsets=offs, M=M, window2=True)
+    yd = dev(y)
+    r = plan.bootstrap_ls(yd, nbs, seed)
+    pw, pk = host(r["power"]), host(r["peak"])
+    for c in (0, 1, 3):
+        assert np.all(pw[c * nbs:(c + 1) * nbs] == 0) and np.all(pk[c * nbs:(c + 1) * nbs] == 0)
+    assert np.all(np.isfinite(pw)) and np.all(pw >= 0) and np.all(pw <= 1 + 1e-4)
+    d = oracle.bootstrap_indices(seed, 300, nbs) + offs[2]
+    ref = oracle.lomb_scargle(t[d[1]], y.astype(np.float64)[d[1]], 1 / 3650.0, M // 2)
+    assert np.max(np.abs(pw[2 * nbs + 1] - ref)) < 2e-5
+    stream = torch.cuda.Stream()
+    out = torch.empty_like(r["power"])
+    pkd = torch.empty_like(r["peak"])
+    kpd = torch.empty_like(r["kpeak"])
+    run = lambda: plan.bootstrap_ls(yd, nbs, seed, stream=stream, out_power=out, out_peak=pkd,
+                                    out_kpeak=kpd)
+    run()
+    stream.synchronize()
+    eager = host(out).copy()
+    g = cu.capture(run, stream)
+    for rep in range(3):
+        out.zero_()
+        torch.cuda.synchronize()
+        g.launch(stream)
+        stream.synchronize()
+        assert np.array_equal(host(out).view(np.uint32), eager.view(np.uint32)), rep
+        plan.bootstrap_ls(yd, nbs, seed + 1)
+        torch.cuda.synchronize()
+    g.close()
+
+
+def test_forward_nfft_single_point_and_empty_curve():
+    x = np.array([0.125], dtype=np.float32)
+    M = 1 << 6
+    g = np.zeros(M, np.complex64)
+    g[M // 2 + 3] = 1.0  # k = 3
+    f = host(cu.NFFTPlan(dev(x), M).forward(dev(g).reshape(1, M)))
+    assert abs(f[0] - np.exp(-2j * np.pi * 3 * 0.125)) < 1e-5
+    # batch with an empty curve
+    offs = np.array([0, 0, 5], dtype=np.int32)
+    x2 = np.linspace(-0.4, 0.4, 5).astype(np.float32)
+    G2 = np.stack([np.zeros(M, np.complex64), g])
+    f2 = host(cu.NFFTPlan(dev(x2), M, offsets=offs).forward(dev(G2)))
+    ref = oracle.ndft_forward(x2.astype(np.float64), g.astype(np.complex128))
+    assert rel_l2(f2, ref) < 1e-5
+
+
 def test_bootstrap_lomb_scargle_rejects_unsupported():
     x, y = uniform_points(1000, 3)
     plan = cu.NFFTPlan(dev(x), 1 << 10)  # no window2
