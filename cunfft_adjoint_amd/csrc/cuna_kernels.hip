@@ -378,16 +378,14 @@ void launch_buckets(const SortWork &W, int K, int *bucket, hipStream_t s) {
 // --------------------------------------------------------------- spread --
 template <typename T, int NTAPS, int VM>
 static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s) {
-    if constexpr (sizeof(T) == 4) {
-        if (!A.gather) {
-            const size_t lds = spread_fx_lds_bytes<T>(A.tlog, NTAPS);
-            require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
-            auto k = k_spread_fx<T, NTAPS, VM>;
-            set_lds(k, lds);
-            hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
-            CUNA_LAUNCH_CHECK();
-            return;
-        }
+    if (!A.gather) {
+        const size_t lds = spread_fx_lds_bytes<T>(A.tlog, NTAPS);
+        require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
+        auto k = k_spread_fx<T, NTAPS, VM>;
+        set_lds(k, lds);
+        hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
+        CUNA_LAUNCH_CHECK();
+        return;
     }
     const size_t lds = spread_lds_bytes<T>(A.tlog, NTAPS, A.cap);
     require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);

@@ -10,10 +10,10 @@
 // boundary classes of its two neighbours and accumulates the taps that land
 // in its tile in LDS, then writes the tile once with coalesced stores.  No
 // global atomics, and both accumulation forms are bitwise reproducible:
-//   k_spread_fx (FP32 default): scatter with 64-bit integer atomics in fixed
+//   k_spread_fx (default): scatter with 64-bit integer atomics in fixed
 //     point (integer sums do not depend on their order);
-//   k_spread (FP64, or FP32 on request): gather, each cell summed by one
-//     thread in a fixed order.
+//   k_spread (on request, CUNA_PLAN_GATHER): gather, each cell summed by
+//     one thread in a fixed order.
 // The fused bootstrap kernel (k_boot_fused) keeps a whole curve's grid in
 // LDS; the backward's gather (k_gather) is the transpose of the spread.
 #pragma once
@@ -606,15 +606,16 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
 }
 
 // ---------------------------------------------- fixed-point scatter --
-// Scatter form for the FP32 library: a thread takes a point, forms its
-// NTAPS weights with three exponentials, and adds them to the tile's LDS
-// cells with 64-bit integer atomics (ds_add_u64) in fixed point.  Integer
-// sums do not depend on the order of the additions, so the result is
-// bitwise reproducible; float LDS atomics would be neither reproducible nor
-// fast on this hardware.  A round of up to CAP points uses the scale 2^s
-// with cnt * max|v| * wscale * 2^s <= 2^62, so no cell can overflow, and a
-// weight keeps at least 2^-37 of the round's largest weight as resolution
-// (float itself keeps 2^-24 of the cell's own sum).
+// Scatter form (both libraries): a thread takes a point, forms its NTAPS
+// weights with three exponentials, and adds them to the tile's LDS cells
+// with 64-bit integer atomics (ds_add_u64) in fixed point.  Integer sums do
+// not depend on the order of the additions, so the result is bitwise
+// reproducible; float LDS atomics would be neither reproducible nor fast on
+// this hardware, and double LDS atomics are compare-and-swap loops.  A round
+// of up to CAP points uses the scale 2^s with cnt * max|v| * wscale * 2^s <=
+// 2^62, so no cell can overflow, and a weight keeps at least 2^-52 of the
+// round's largest weight as resolution (float itself keeps 2^-24 of the
+// cell's own sum, double 2^-53).
 // Split of a scaled weight x (|x| < 2^62) into a two's-complement 64-bit
 // integer: h = floor(x / 2^32) and l = x - h 2^32 in [0, 2^32).  Exact for
 // x >= 0; for -2^31 < x < 0, l is rounded to a multiple of 2^8, an error far
@@ -624,16 +625,26 @@ __device__ __forceinline__ unsigned long long to_fixed64(float x) {
     const float l = fminf(fmaf(h, -0x1p32f, x), 4294967040.0f);
     return ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)l;
 }
+// The same in double: l is exact, and is rounded to the nearest integer
+// (at most 2^32 - 1, so a tiny negative x gives -1, one unit of the scale).
+__device__ __forceinline__ unsigned long long to_fixed64(double x) {
+    const double h = floor(x * 0x1p-32);
+    const double l = fmin(rint(fma(h, -0x1p32, x)), 4294967295.0);
+    return ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)l;
+}
 
 template <typename T, int NTAPS, int VM>
 __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
-    static_assert(sizeof(T) == 4, "fixed-point scatter is the FP32 spread");
     constexpr int NT = 256, PPT = 4, CAP = NT * PPT, Q0 = (NTAPS - 1) / 2, HALO = NTAPS;
     const int TT = 1 << A.tlog;
     const int NC = TT + 2 * HALO;  // cells -HALO .. TT+HALO-1
     unsigned long long *acc = reinterpret_cast<unsigned long long *>(cuna_smem);
-    float *facc = reinterpret_cast<float *>(acc + NC);  // TT, for tiles of several rounds
-    __shared__ float red[NT / 64];
+    T *facc = reinterpret_cast<T *>(acc + NC);  // TT, for tiles of several rounds
+    __shared__ T red[NT / 64];
+    // finite values whose products with cnt and wscale stay finite
+    constexpr T kBig = sizeof(T) == 4 ? T(3.0e38f) : T(1.0e300);
+    // 2^sexp also keeps wscale * 2^sexp finite
+    constexpr int kMaxExp = sizeof(T) == 4 ? 100 : 1000;
 
     // consecutive tiles go to different XCDs (hardware round-robin): dense
     // regions of clustered points are spread over all of them
@@ -645,11 +656,15 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
     const int total = R.total;
     const bool multi = total > CAP;
     const int tid = threadIdx.x;
-    const float c1 = A.e3[Q0 + 1];  // exp(-bi): e3[q] = exp(-(q - Q0)^2 bi)
+    const T c1 = A.e3[Q0 + 1];  // exp(-bi): e3[q] = exp(-(q - Q0)^2 bi)
 
     for (int g = tid; g < NC; g += NT) acc[g] = 0ull;
     if (multi)
-        for (int g = tid; g < TT; g += NT) facc[g] = 0.f;
+        for (int g = tid; g < TT; g += NT) facc[g] = T(0);
+    // the cells a thread reads back were cleared by other threads: a tile
+    // with no points has no other barrier before the final read, and would
+    // otherwise see what an earlier workgroup left in this LDS
+    __syncthreads();
     bool finite = true;
     int sexp = 0;
     for (int base = 0; base < total; base += CAP) {
@@ -658,7 +673,7 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
         // first point and is masked afterwards, so that all PPT slots'
         // loads issue together (a branch per slot would wait out each one).
         int am[PPT], pk[PPT];
-        float ev[PPT], vv[PPT];
+        T ev[PPT], vv[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const int j = base + k * NT + tid;
@@ -669,44 +684,44 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
             ev[k] = pr.e;
             pk[k] = A.perm[s];
         }
-        float vmax = 0.f;
+        T vmax = T(0);
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             T v, e = ev[k];
             point_value<T, VM>(A, pk[k], c, rep, v, e);
             const bool ok = base + k * NT + tid < base + cnt;
-            vv[k] = ok ? v : 0.f;
+            vv[k] = ok ? v : T(0);
             ev[k] = e;
             // a non-finite value or position (NaN x gives NaN e) makes the
             // round's maximum infinite, and the tile's output NaN
-            float av = fabsf(vv[k]);
-            if (!(av <= 3.0e38f) || (ok && !(fabsf(e) <= 3.0e38f))) av = INFINITY;
-            vmax = fmaxf(vmax, av);
+            T av = fabs(vv[k]);
+            if (!(av <= kBig) || (ok && !(fabs(e) <= kBig))) av = T(INFINITY);
+            vmax = fmax(vmax, av);
         }
         // round maximum of |v|
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
         if ((tid & 63) == 0) red[tid >> 6] = vmax;
         __syncthreads();
         vmax = red[0];
 #pragma unroll
-        for (int w = 1; w < NT / 64; ++w) vmax = fmaxf(vmax, red[w]);
-        if (!(vmax <= 3.0e38f)) finite = false;
-        // 2^sexp: cnt * vmax * wscale * 2^sexp <= 2^62, and wscale * 2^sexp <= 2^100
-        const float bound = (float)cnt * vmax * A.wscale;
+        for (int w = 1; w < NT / 64; ++w) vmax = fmax(vmax, red[w]);
+        if (!(vmax <= kBig)) finite = false;
+        // 2^sexp: cnt * vmax * wscale * 2^sexp <= 2^62, and wscale * 2^sexp <= 2^kMaxExp
+        const T bound = (T)cnt * vmax * A.wscale;
         int eb = 0;
-        if (bound > 0.f && finite) (void)frexpf(bound, &eb);
+        if (bound > T(0) && finite) (void)frexp(bound, &eb);
         else eb = -64;
         int ew = 0;
-        (void)frexpf(A.wscale, &ew);
-        sexp = min(62 - eb, 100 - ew);
-        const float mul = ldexpf(A.wscale, sexp);
+        (void)frexp(A.wscale, &ew);
+        sexp = min(62 - eb, kMaxExp - ew);
+        const T mul = ldexp(A.wscale, sexp);
         if (finite) {
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
                 if (base + k * NT + tid < base + cnt) {
-                    float w[NTAPS];
-                    tap_weights_rec<float, NTAPS>(vv[k] * mul, ev[k], A.bi, c1, w);
+                    T w[NTAPS];
+                    tap_weights_rec<T, NTAPS>(vv[k] * mul, ev[k], A.bi, c1, w);
                     const int c0 = am[k] - Q0;  // first tap's cell
                     if (R.tiny) {
 #pragma unroll
@@ -728,9 +743,9 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
             // fold the round into facc; each thread clears the cells it read
             // (a different thread-to-cell map for the clear would race with
             // the reads of other waves), and the halo cells no thread reads
-            const float inv = ldexpf(1.f, -sexp);
+            const T inv = ldexp(T(1), -sexp);
             for (int g = tid; g < TT; g += NT) {
-                facc[g] += (float)(long long)acc[HALO + g] * inv;
+                facc[g] += (T)(long long)acc[HALO + g] * inv;
                 acc[HALO + g] = 0ull;
             }
             for (int h = tid; h < 2 * HALO; h += NT) acc[h < HALO ? h : TT + h] = 0ull;
@@ -739,17 +754,17 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
     }
     T *gout = A.grid + (long)slab * A.grid_stride + R.tstart;
     const int tlen = min(TT, A.G - R.tstart);
-    const float inv = ldexpf(1.f, -sexp);
+    const T inv = ldexp(T(1), -sexp);
     for (int g = tid; g < tlen; g += NT) {
-        float v = multi ? facc[g] : (float)(long long)acc[HALO + g] * inv;
-        gout[g] = finite ? v : NAN;
+        const T v = multi ? facc[g] : (T)(long long)acc[HALO + g] * inv;
+        gout[g] = finite ? v : T(NAN);
     }
 }
 
 template <typename T>
 __host__ __device__ inline size_t spread_fx_lds_bytes(int tlog, int ntaps) {
     const size_t tt = (size_t)1 << tlog;
-    return sizeof(unsigned long long) * (tt + 2 * (size_t)ntaps) + sizeof(float) * tt;
+    return sizeof(unsigned long long) * (tt + 2 * (size_t)ntaps) + sizeof(T) * tt;
 }
 
 // ------------------------------------------- fused batched bootstrap --

@@ -37,9 +37,9 @@ typedef struct cuna_nfft_plan_s *cuna_nfft_plan;
 #define CUNA_PLAN_WINDOW2 0x1 /* also bin the points 2x (window transform at 2f) */
 #define CUNA_PLAN_UNFUSED 0x2 /* bootstrap through spread + batched FFT kernels instead of
                                  the fused single-launch kernel (grids <= 2^15 cells) */
-#define CUNA_PLAN_GATHER 0x4  /* FP32 library: spread in gather form (each cell sums the
-                                 points that reach it) instead of the default scatter with
-                                 fixed-point integer LDS atomics; both are deterministic */
+#define CUNA_PLAN_GATHER 0x4  /* spread in gather form (each cell sums the points that
+                                 reach it) instead of the default scatter with fixed-point
+                                 integer LDS atomics; both are deterministic */
 
 typedef struct {
     int mode;      /* CUNA_MODE_ACCURATE (default) or CUNA_MODE_COMPAT */
