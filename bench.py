@@ -450,6 +450,141 @@ def bench_precision(torch, cu):
     }
 
 
+def bench_c2_variants(x, y, torch, cu, K=50):
+    """The N=1e6 clustered, M=2^20 workload through the other entry points:
+    the FP64 library (accurate, m=13) and the reference's compat formula
+    (cunfft_adjoint_raw_async, ng=2^21) in both libraries.  K transforms
+    replayed from one graph, timed with HIP events on the launch stream."""
+    from tests._data import rel_l2
+    import oracle
+
+    N, M = x.size, 1 << 20
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    ev = HipEvents()
+
+    def graph_ms(fn):
+        fn()
+        stream.synchronize()
+        g = cu.capture(lambda: [fn() for _ in range(K)], stream)
+        g.launch(stream)
+        stream.synchronize()
+        e = ev.create(2)
+        ev.record(e[0], sp)
+        g.launch(stream)
+        ev.record(e[1], sp)
+        stream.synchronize()
+        ms = ev.ms(e[0], e[1]) / K
+        ev.destroy(e)
+        g.close()
+        return ms
+
+    rng = np.random.default_rng(1)
+    k0 = np.concatenate([[-M // 2, -16], rng.integers(-M // 2, M // 2 - 32, 6)]).astype(np.int64)
+    ref = oracle.ndft_bands(x.astype(np.float64), y.astype(np.float64), k0, 32)
+    res = {}
+    xd64 = torch.from_numpy(x.astype(np.float64)).cuda()
+    yd64 = torch.from_numpy(y.astype(np.float64)).cuda()
+    plan = cu.NFFTPlan(xd64, M)
+    out = torch.empty((1, M), dtype=torch.complex128, device="cuda")
+    ms = graph_ms(lambda: plan.adjoint(yd64, out, stream=stream))
+    st = ev.create(4)
+    arr = (ctypes.c_void_p * 4)(*st)
+    plan.L.cuna_nfft_set_profile_events(plan.h, arr, 4)
+    plan.adjoint(yd64, out, stream=stream)
+    stream.synchronize()
+    plan.L.cuna_nfft_set_profile_events(plan.h, None, 0)
+    stages = {"spread": ev.ms(st[0], st[1]), "fft pass 1": ev.ms(st[1], st[2]),
+              "fft pass 2 + deconvolution": ev.ms(st[2], st[3])}
+    ev.destroy(st)
+    got = out[0].cpu().numpy()
+    sel = np.stack([got[k + M // 2:k + M // 2 + 32] for k in k0])
+    res["fp64_accurate"] = {"ms_per_transform": ms, "adj_nfft_per_s": 1e3 / ms, "m": plan.info().m,
+                            "rel_err_vs_ndft": rel_l2(sel, ref), "stages_ms_one_transform": stages}
+    plan.close()
+    del out
+    ng = 1 << 21
+    for prec, xd, yd in (("fp32", torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()),
+                         ("fp64", xd64, yd64)):
+        fp = cu.generate_filter_properties(xd, N, ng)
+        fh = torch.zeros(ng, dtype=torch.complex64 if prec == "fp32" else torch.complex128,
+                         device="cuda")
+        ms = graph_ms(lambda: cu.cunfft_adjoint_raw_async(xd, yd, fh, N, ng, fp, stream=stream))
+        res[prec + "_compat_raw_async"] = {"ms_per_transform": ms, "adj_nfft_per_s": 1e3 / ms}
+        fp.free()
+        del fh
+    res["workload"] = ("N=1e6 clustered, M=2^20 (accurate, sigma=2) / ng=2^21 (compat, the "
+                       "reference's formula through cunfft_adjoint_raw_async); plan reused")
+    return res
+
+
+def bench_compat_batch(torch, cu, nlc=256, K=5):
+    """The reference formula over many curves: nlc light curves of
+    log-uniform length in [100, 5e4], each with ng = the next power of two
+    above its length, through one cunfft_adjoint_raw_batched_async call
+    (cached multi-curve plan: one spread + one FFT launch per distinct ng)
+    against nlc cunfft_adjoint_raw_async calls; both replayed from graphs."""
+    from tests._data import rel_l2
+    import oracle
+
+    rng = np.random.default_rng(77)
+    n = np.exp(rng.uniform(np.log(100), np.log(5e4), nlc)).astype(np.int64)
+    ng = [1 << int(np.ceil(np.log2(v + 1))) for v in n]
+    n = [int(v) for v in n]
+    off = np.concatenate([[0], np.cumsum(n)])
+    x = rng.uniform(-0.5, 0.5, off[-1]).astype(np.float32)
+    y = rng.standard_normal(off[-1]).astype(np.float32)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    idist = max(ng)
+    fps = [cu.generate_filter_properties(xd[off[i]:off[i + 1]], n[i], ng[i]) for i in range(nlc)]
+    fh = torch.zeros(nlc * idist, dtype=torch.complex64, device="cuda")
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    ev = HipEvents()
+
+    def graph_ms(fn):
+        fn()
+        stream.synchronize()
+        g = cu.capture(fn, stream)
+        g.launch(stream)
+        stream.synchronize()
+        e = ev.create(2)
+        ev.record(e[0], sp)
+        for _ in range(K):
+            g.launch(stream)
+        ev.record(e[1], sp)
+        stream.synchronize()
+        ms = ev.ms(e[0], e[1]) / K
+        ev.destroy(e)
+        g.close()
+        return ms
+
+    batched = lambda: cu.cunfft_adjoint_raw_batched_async(xd, yd, fh, n, ng, fps, stream=stream)
+    ms_b = graph_ms(batched)
+    got = fh.cpu().numpy().reshape(nlc, idist)
+    err = 0.0
+    for i in (0, nlc // 2, nlc - 1):
+        sl = slice(off[i], off[i + 1])
+        err = max(err, rel_l2(got[i, :ng[i]], oracle.compat_adjoint(x[sl], y[sl], ng[i], False)))
+    views = [(xd[off[i]:off[i + 1]], yd[off[i]:off[i + 1]], fh[i * idist:i * idist + ng[i]])
+             for i in range(nlc)]
+
+    def loop():
+        for i, (xv, yv, fv) in enumerate(views):
+            cu.cunfft_adjoint_raw_async(xv, yv, fv, n[i], ng[i], fps[i], stream=stream)
+
+    ms_l = graph_ms(loop)
+    for f in fps:
+        f.free()
+    cu.batch_cache_clear("f")
+    return {
+        "workload": f"{nlc} curves, n log-uniform in [100, 5e4] ({off[-1]} points), ng = next "
+                    f"power of two above n ({len(set(ng))} distinct), FP32 compat formula",
+        "batched_ms": ms_b, "per_curve_calls_ms": ms_l, "speedup": ms_l / ms_b,
+        "curves_per_s_batched": nlc / (ms_b * 1e-3), "max_rel_err_vs_oracle": err,
+    }
+
+
 def bench_differentiation(torch, cu, steps=20):
     """64 curves x N = 2e4, M = 2^16, L = sum_k w_k |F_k|^2: forward (adjoint)
     and backward (d/dy and d/dx) times of the batch, each replayed as a graph."""
@@ -589,6 +724,8 @@ def main(argv=None):
         res["bootstrap"] = bench_bootstrap(args, torch, cu)
     if not args.no_extras:
         cu.load("d")
+        res["c2_variants"] = bench_c2_variants(r["x"], r["y"], torch, cu)
+        res["compat_batch"] = bench_compat_batch(torch, cu)
         res["precision"] = bench_precision(torch, cu)
         res["differentiation"] = bench_differentiation(torch, cu)
     if not args.no_cpu:

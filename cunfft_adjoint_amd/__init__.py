@@ -16,9 +16,12 @@ from .api import (  # noqa: F401
     cunfft_adjoint_raw,
     cunfft_adjoint_raw_async,
     cunfft_adjoint_raw_async_bootstrap,
+    cunfft_adjoint_raw_batched_async,
+    batch_cache_clear,
     fft,
     generate_filter_properties,
     generate_pinned_filter_properties,
+    generate_pinned_filter_properties_batch,
 )
 
 __version__ = "0.1.0"

@@ -85,6 +85,10 @@ def _protos(prec: str):
         "cunfft_adjoint_raw": (None, [P, P, P, c_int, c_int, P]),
         "cunfft_adjoint_raw_async": (None, [P, P, P, c_int, c_int, P, P]),
         "cunfft_adjoint_raw_async_bootstrap": (None, [P, P, P, c_int, c_int, c_int, P, P, c_uint]),
+        # include/cuna_nfft.h: the reference's commented-out batched API
+        "cunfft_adjoint_raw_batched_async": (None, [P, P, P, P, P, c_int, P, P]),
+        "generate_pinned_filter_properties_batch": (None, [P, P, P, c_int, P, P, P]),
+        "cuna_batch_cache_clear": (None, []),
         # include/cuna_typedefs.h
         "init_plan": (None, [P, RP, RP, c_int, c_int, c_uint]),
         "free_plan": (None, [P]),

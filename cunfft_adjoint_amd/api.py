@@ -154,6 +154,51 @@ def cunfft_adjoint_raw_async_bootstrap(x, f_data, f_hat, n: int, ng: int, nbs: i
     check(L)
 
 
+def cunfft_adjoint_raw_batched_async(x, f_data, f_hat, n, ng, fprops=None, stream=None):
+    """Batch of curves through the reference formula (the batched transform
+    the reference keeps commented out, src/cuna_kernel.cu:302-358): curve i
+    has n[i] points of the concatenated x / f_data and ng[i] cells; its
+    spectrum fills the first ng[i] values of slab i of f_hat (slab stride
+    max(ng)).  fprops: per-curve FilterProperties (or device pointers) from
+    generate_filter_properties; when given, the multi-curve plan is cached."""
+    L = load(_prec_of(x))
+    nlc = len(n)
+    if len(ng) != nlc:
+        raise ValueError("n and ng must have the same length")
+    na = (ctypes.c_int * nlc)(*[int(v) for v in n])
+    ga = (ctypes.c_int * nlc)(*[int(v) for v in ng])
+    fa = None
+    if fprops is not None:
+        if len(fprops) != nlc:
+            raise ValueError("one filter_properties per curve")
+        fa = (ctypes.c_void_p * nlc)(
+            *[f.device if isinstance(f, FilterProperties) else f for f in fprops])
+    L.cunfft_adjoint_raw_batched_async(_ptr(x), _ptr(f_data), _ptr(f_hat), na, ga, nlc, fa,
+                                       _stream(stream))
+    check(L)
+
+
+def generate_pinned_filter_properties_batch(x, n, ng, h_fprops, d_fprops, stream=None):
+    """src/gaussian_filter.cu:67-99 (commented out in the reference): h_fprops
+    are ctypes filter_properties with caller-allocated E1/E2/E3, d_fprops
+    device pointers, one each per curve."""
+    L = load(_prec_of(x))
+    nlc = len(n)
+    na = (ctypes.c_int * nlc)(*[int(v) for v in n])
+    ga = (ctypes.c_int * nlc)(*[int(v) for v in ng])
+    ha = (ctypes.c_void_p * nlc)(*[ctypes.addressof(h) for h in h_fprops])
+    da = (ctypes.c_void_p * nlc)(*d_fprops)
+    L.generate_pinned_filter_properties_batch(_ptr(x), na, ga, nlc, ha, da, _stream(stream))
+    check(L)
+
+
+def batch_cache_clear(prec: str = "f"):
+    """Frees the cached multi-curve plans of one library."""
+    L = load(prec)
+    L.cuna_batch_cache_clear()
+    check(L)
+
+
 class CunaPlan:
     """init_plan / cunfft_adjoint_from_plan / free_plan (inc/cuna_typedefs.h:151-155,
     inc/cuna.h:41) over host arrays."""

@@ -2,7 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <cmath>
 #include <cstring>
+#include <list>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 
@@ -70,6 +74,19 @@ __global__ void k_scale2(const T *in, T *out, long n) {
 __global__ void k_fill_int(int *p, long n, int v) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// dst[i] = src[idx[i]]
+template <typename T>
+__global__ void k_take(const T *src, const int *idx, long n, T *dst) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+// perm[i] = idx[perm[i]]: sorted position -> index into the caller's arrays
+__global__ void k_compose(int *perm, const int *idx, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[i] = idx[perm[i]];
 }
 
 // ------------------------------------------------------------ TiledPoints --
@@ -212,6 +229,7 @@ SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
     A.gather = (spec.flags & CUNA_PLAN_GATHER) ? 1 : 0;
     A.bi = T(1) / beta;
     A.wscale = wscale;
+    A.cpar = cpar_c;
     for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
     return A;
 }
@@ -241,6 +259,8 @@ void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s, hipEv
         sk.invG = T(1) / (T)G;
         sk.qb = b_c * T(0.25);
         sk.normfac = normfac_c;
+        sk.row_off = row_off_c;
+        sk.qb_row = qb_row_c;
         fft_real_forward<T>(sh, grid_ptr, sk, s, mid);
     }
 }
@@ -717,6 +737,190 @@ static void generate_impl(const R *x, int n, int ng, filter_properties *h, filte
     reg_put(d, std::move(p));
 }
 
+// ------------------------------------------------ batches of curves --
+// The reference's batched compat transform, which it leaves commented out
+// (src/cuna_kernel.cu:302-358 with fast_gridding_batched, src/gridding.cu:
+// 119-149, and normalize_batch, src/gaussian_filter.cu:303-318): curve i has
+// n[i] points of the concatenated x (and f_data) and ng[i] grid cells; its
+// spectrum fills the first ng[i] values of slab i of f_hat, slab stride
+// idist = max ng.  Curves with the same ng share one plan: one spread launch
+// over all of them, each curve with its own filter width b, and one batched
+// FFT whose deconvolution also takes b per curve.
+struct CompatBatch {
+    const void *x_key = nullptr;
+    std::vector<int> n, ng;
+    std::vector<const void *> fps;  // device filter_properties per curve
+    int idist = 0;
+    struct Group {
+        std::unique_ptr<Plan<R>> plan;
+        DevMem cpar, qb_row, row_off;
+    };
+    std::vector<std::unique_ptr<Group>> groups;
+
+    void run(const R *y, cplx<R> *f_hat, hipStream_t s) {
+        for (auto &g : groups) g->plan->adjoint(y, f_hat, s);
+    }
+};
+
+static std::unique_ptr<CompatBatch> make_compat_batch(const R *x, const int *n, const int *ng,
+                                                      int nlc, hipStream_t s) {
+    require(nlc > 0, "nlc must be positive");
+    require(x != nullptr && n != nullptr && ng != nullptr, "x, n and ng must not be NULL");
+    auto B = std::make_unique<CompatBatch>();
+    B->x_key = x;
+    B->n.assign(n, n + nlc);
+    B->ng.assign(ng, ng + nlc);
+    std::vector<long> off(nlc + 1, 0);
+    for (int i = 0; i < nlc; ++i) {
+        check_compat_sizes(n[i], ng[i]);
+        require(ilog2(ng[i]) - 1 <= max_fft_lg<R>(), "ng too large", CUNA_ERROR_UNSUPPORTED);
+        off[i + 1] = off[i] + n[i];
+        B->idist = std::max(B->idist, ng[i]);
+    }
+    require(off[nlc] <= (long)INT_MAX, "too many points");
+    std::map<int, std::vector<int>> by_ng;
+    for (int i = 0; i < nlc; ++i) by_ng[ng[i]].push_back(i);
+    const double pi = 3.14159265358979323846;
+    for (const auto &kv : by_ng) {
+        const int g = kv.first;
+        const std::vector<int> &cur = kv.second;
+        const int nc = (int)cur.size();
+        // the group's points in curve order: their indices in x, and offsets
+        std::vector<int> gidx, goff(1, 0);
+        for (int i : cur) {
+            for (long j = off[i]; j < off[i + 1]; ++j) gidx.push_back((int)j);
+            goff.push_back((int)gidx.size());
+        }
+        const long gn = (long)gidx.size();
+        DevBuf d_idx, d_x;
+        d_idx.alloc(sizeof(int) * gn, s);
+        d_x.alloc(sizeof(R) * gn, s);
+        CUNA_HIP(hipMemcpyAsync(d_idx.ptr, gidx.data(), sizeof(int) * gn, hipMemcpyHostToDevice,
+                                s));
+        hipLaunchKernelGGL(k_take<R>, dim3(ceil_div(gn, 256)), dim3(256), 0, s, x,
+                           d_idx.as<int>(), gn, d_x.as<R>());
+        CUNA_LAUNCH_CHECK();
+        // one compat plan over the group's curves; the filter constants of
+        // its first curve fill the scalar fields, the per-curve arrays below
+        // take their place in the kernels
+        auto P = std::make_unique<Plan<R>>();
+        P->spec.mode = CUNA_MODE_COMPAT;
+        P->M = g;
+        P->G = g;
+        P->lgN = ilog2(g) - 1;
+        compat_constants<R>(n[cur[0]], g, P->b_c, P->normfac_c, P->R_f);
+        P->ntaps = 2 * P->R_f - 1;
+        P->olo = -(P->R_f - 1);
+        P->beta = P->b_c;
+        P->wscale = (R)(1.0 / std::sqrt(pi * (double)P->b_c));
+        for (int q = 0; q < P->ntaps; ++q) {
+            const double o = (double)(q + P->olo);
+            P->e3[q] = (R)std::exp(-o * o / (double)P->b_c);
+        }
+        SetupArgs<R> A{};
+        A.x = d_x.as<R>();
+        A.mode = SETUP_COMPAT;
+        A.R_f = P->R_f;
+        A.b = P->b_c;
+        A.e1s = P->wscale;
+        P->pts.build(A, goff, g, default_tlog<R>(), P->ntaps, false, false, s);
+        hipLaunchKernelGGL(k_compose, dim3(ceil_div(std::max(gn, 1L), 256)), dim3(256), 0, s,
+                           P->pts.perm.as<int>(), d_idx.as<int>(), gn);
+        CUNA_LAUNCH_CHECK();
+        // per curve: (1/b, 1/sqrt(pi b), exp(-1/b)) as the single-curve plan
+        // forms them, b/4 for the deconvolution, and the output slab
+        std::vector<R> cp(3 * (size_t)nc), qb(nc);
+        std::vector<long> ro(nc);
+        for (int k = 0; k < nc; ++k) {
+            R b, nf;
+            int rf;
+            compat_constants<R>(n[cur[k]], g, b, nf, rf);
+            cp[3 * k] = R(1) / b;
+            cp[3 * k + 1] = (R)(1.0 / std::sqrt(pi * (double)b));
+            cp[3 * k + 2] = (R)std::exp(-1.0 / (double)b);
+            qb[k] = b * R(0.25);
+            ro[k] = (long)cur[k] * B->idist;
+        }
+        auto G = std::make_unique<CompatBatch::Group>();
+        G->cpar.alloc(sizeof(R) * cp.size());
+        G->qb_row.alloc(sizeof(R) * qb.size());
+        G->row_off.alloc(sizeof(long) * ro.size());
+        CUNA_HIP(hipMemcpyAsync(G->cpar.ptr, cp.data(), sizeof(R) * cp.size(),
+                                hipMemcpyHostToDevice, s));
+        CUNA_HIP(hipMemcpyAsync(G->qb_row.ptr, qb.data(), sizeof(R) * qb.size(),
+                                hipMemcpyHostToDevice, s));
+        CUNA_HIP(hipMemcpyAsync(G->row_off.ptr, ro.data(), sizeof(long) * ro.size(),
+                                hipMemcpyHostToDevice, s));
+        P->cpar_c = G->cpar.as<R>();
+        P->qb_row_c = G->qb_row.as<R>();
+        P->row_off_c = G->row_off.as<long>();
+        G->plan = std::move(P);
+        B->groups.push_back(std::move(G));
+        // the host tables above are read by the copies in stream order
+        CUNA_HIP(hipStreamSynchronize(s));
+    }
+    return B;
+}
+
+// Multi-curve plan cache: a batch whose curves all have registered filter
+// properties (generate_*filter_properties*) is kept, keyed by x, the sizes
+// and those pointers, until one of them is freed; at most kBatchCacheMax
+// batches, the least recently used dropped first.  Batches with unregistered
+// or NULL filter properties are rebuilt on every call.
+namespace {
+constexpr size_t kBatchCacheMax = 16;
+std::mutex g_batch_mu;
+std::list<std::shared_ptr<CompatBatch>> g_batches;  // most recently used first
+}  // namespace
+
+static std::shared_ptr<CompatBatch> batch_for(const R *x, const int *n, const int *ng, int nlc,
+                                              const filter_properties *const *fps,
+                                              hipStream_t s) {
+    bool cacheable = fps != nullptr && nlc > 0 && n != nullptr && ng != nullptr;
+    for (int i = 0; cacheable && i < nlc; ++i) cacheable = fps[i] && reg_get(fps[i]);
+    if (cacheable) {
+        std::lock_guard<std::mutex> lk(g_batch_mu);
+        for (auto it = g_batches.begin(); it != g_batches.end(); ++it) {
+            const CompatBatch &b = **it;
+            if (b.x_key == x && (int)b.n.size() == nlc &&
+                std::equal(b.n.begin(), b.n.end(), n) && std::equal(b.ng.begin(), b.ng.end(), ng) &&
+                std::equal(b.fps.begin(), b.fps.end(), fps)) {
+                auto hit = *it;
+                g_batches.erase(it);
+                g_batches.push_front(hit);
+                return hit;
+            }
+        }
+    }
+    std::shared_ptr<CompatBatch> fresh(make_compat_batch(x, n, ng, nlc, s));
+    if (cacheable) {
+        fresh->fps.assign(fps, fps + nlc);
+        std::shared_ptr<CompatBatch> victim;
+        std::lock_guard<std::mutex> lk(g_batch_mu);
+        g_batches.push_front(fresh);
+        if (g_batches.size() > kBatchCacheMax) {
+            victim = g_batches.back();
+            g_batches.pop_back();
+        }
+    }
+    return fresh;
+}
+
+// drops the cached batches that use the filter properties d (all if NULL)
+static void batch_drop(const void *d) {
+    std::list<std::shared_ptr<CompatBatch>> victims;
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    for (auto it = g_batches.begin(); it != g_batches.end();) {
+        const auto &f = (*it)->fps;
+        if (!d || std::find(f.begin(), f.end(), d) != f.end()) {
+            victims.push_back(*it);
+            it = g_batches.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
 }  // namespace cuna
 
 using namespace cuna;
@@ -774,6 +978,7 @@ CUNA_API void generate_pinned_filter_properties(const dTyp *x, const int n, cons
 
 CUNA_API void free_filter_properties(filter_properties *d_fp, filter_properties *fp) {
     guarded([&] {
+        if (d_fp) batch_drop(d_fp);
         reg_drop(d_fp);
         if (fp) {
             if (fp->E1) CUNA_HIP(hipFree(fp->E1));
@@ -827,6 +1032,40 @@ CUNA_API void cunfft_adjoint_raw_async_bootstrap(const dTyp *x, const dTyp *f_da
         auto p = compat_plan_for(x, n, ng, gpu_fprops, stream);
         p->bootstrap_compat(f_data, reinterpret_cast<cplx<R> *>(f_hat), nbs, seed, stream);
     });
+}
+
+CUNA_API void cunfft_adjoint_raw_batched_async(const dTyp *x, const dTyp *f_data,
+                                               Complex *f_hat, const int *n, const int *ng,
+                                               const int nlc,
+                                               const filter_properties *const *gpu_fprops,
+                                               hipStream_t stream) {
+    guarded([&] {
+        require(f_hat != nullptr, "f_hat is NULL");
+        auto b = batch_for(x, n, ng, nlc, gpu_fprops, stream);
+        b->run(f_data, reinterpret_cast<cplx<R> *>(f_hat), stream);
+    });
+}
+
+CUNA_API void generate_pinned_filter_properties_batch(const dTyp *x, const int *n, const int *ng,
+                                                      const int nlc, filter_properties **h_fprops,
+                                                      filter_properties **d_fprops,
+                                                      hipStream_t stream) {
+    guarded([&] {
+        require(nlc > 0, "nlc must be positive");
+        require(x && n && ng && h_fprops && d_fprops, "NULL argument");
+        long off = 0;
+        for (int i = 0; i < nlc; ++i) {
+            require(h_fprops[i] && h_fprops[i]->E1 && h_fprops[i]->E2 && h_fprops[i]->E3,
+                    "h_fprops[i] and its E1/E2/E3 arrays must be allocated by the caller");
+            check_compat_sizes(n[i], ng[i]);
+            generate_impl(x + off, n[i], ng[i], h_fprops[i], d_fprops[i], stream, false);
+            off += n[i];
+        }
+    });
+}
+
+CUNA_API void cuna_batch_cache_clear(void) {
+    guarded([&] { batch_drop(nullptr); });
 }
 
 // ------------------------------------------------------------ plan path --

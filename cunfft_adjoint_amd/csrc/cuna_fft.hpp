@@ -346,6 +346,9 @@ struct SinkAccurate {
     long M;
     long ostride;  // per batch
     T invG, coef, pref;
+    // the sink of batch row b and where that row's output starts
+    __device__ __forceinline__ const SinkAccurate &for_row(long) const { return *this; }
+    __device__ __forceinline__ cplx<T> *row_out(long b) const { return out + b * ostride; }
     __device__ __forceinline__ void operator()(cplx<T> *o, long f, cplx<T> A) const {
         if (f <= M / 2) {
             T kk = (T)f * invG;
@@ -364,6 +367,18 @@ struct SinkCompat {
     long G;
     long ostride;
     T invG, qb, normfac;  // qb = b/4
+    // batches of curves with their own b (each nullable): output offset and
+    // b/4 of batch row b
+    const long *row_off;
+    const T *qb_row;
+    __device__ __forceinline__ SinkCompat for_row(long b) const {
+        SinkCompat r = *this;
+        if (qb_row) r.qb = qb_row[b];
+        return r;
+    }
+    __device__ __forceinline__ cplx<T> *row_out(long b) const {
+        return out + (row_off ? row_off[b] : b * ostride);
+    }
     __device__ __forceinline__ T fac(long r) const {
         T K = (T)r * invG;
         return normfac * dexp(K * K * qb);
@@ -491,14 +506,15 @@ __global__ __launch_bounds__(NT) void k_fft_small(Loader ld, Sink sk, cplx<T> *p
         lds_fft<T, S, LG, 1, NT>(lds, tw, fetch);
     }
     if constexpr (MODE == 0) {
-        cplx<T> *o = sk.out + b * sk.ostride;
+        const auto skb = sk.for_row(b);
+        cplx<T> *o = sk.row_out(b);
 #pragma unroll
         for (int i = 0; i < OPT; ++i) {
             const int r = tid + i * NT;
             if (r <= N / 2) {
                 cplx<T> Zr = lds[lds_sw(r)];
                 cplx<T> Zn = lds[lds_sw((N - r) & (N - 1))];
-                emit_pair(sk, o, (long)r, (long)N, Zr, Zn, ot[i].get());
+                emit_pair(skb, o, (long)r, (long)N, Zr, Zn, ot[i].get());
             }
         }
     } else {
@@ -611,7 +627,8 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
     } else {
         lds_fft<T, 1, LG1, W, NT>(lds, tw, fetch);
     }
-    cplx<T> *o = sk.out + b * sk.ostride;
+    const auto skb = sk.for_row(b);
+    cplx<T> *o = sk.row_out(b);
     // pairs: slot h, element r1 of the primary row with its mirror element
 #pragma unroll
     for (int k = 0; k < OPT; ++k) {
@@ -623,21 +640,21 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
             long r = rho + (long)L2 * r1;
             cplx<T> Zr = lds[lds_sw(r1 * W + h)];
             cplx<T> Zn = lds[lds_sw((L1 - 1 - r1) * W + (H + h))];
-            emit_pair(sk, o, r, N, Zr, Zn, ot[k].get());
+            emit_pair(skb, o, r, N, Zr, Zn, ot[k].get());
         } else {
             // row 0 (slot h=0): element r1 pairs with (L1 - r1) mod L1, r1 <= L1/2
             if (r1 <= L1 / 2) {
                 long r = (long)L2 * r1;
                 cplx<T> Zr = lds[lds_sw(r1 * W + 0)];
                 cplx<T> Zn = lds[lds_sw(((L1 - r1) & (L1 - 1)) * W + 0)];
-                emit_pair(sk, o, r, N, Zr, Zn, ot[k].get());
+                emit_pair(skb, o, r, N, Zr, Zn, ot[k].get());
             }
             // row L2/2 (slot H): element r1 pairs with L1-1-r1, r1 < L1/2
             if (r1 < L1 / 2) {
                 long r = L2 / 2 + (long)L2 * r1;
                 cplx<T> Zr = lds[lds_sw(r1 * W + H)];
                 cplx<T> Zn = lds[lds_sw((L1 - 1 - r1) * W + H)];
-                emit_pair(sk, o, r, N, Zr, Zn, twG(r));
+                emit_pair(skb, o, r, N, Zr, Zn, twG(r));
             }
         }
     }

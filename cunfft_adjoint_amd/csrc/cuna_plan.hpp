@@ -78,6 +78,12 @@ struct Plan {
     // compat constants (src/gaussian_filter.cu:54-57)
     T b_c = 0, normfac_c = 0;
     int R_f = 0;
+    // compat plans over several curves with their own b (device arrays, set
+    // by the batched transform): per curve (1/b, 1/sqrt(pi b), exp(-1/b)),
+    // b/4, and the output offset of the curve's spectrum
+    const T *cpar_c = nullptr;
+    const T *qb_row_c = nullptr;
+    const long *row_off_c = nullptr;
     // kept to identify the x a registered plan belongs to
     const void *x_key = nullptr;
     TiledPoints<T> pts, pts2;  // pts2: points 2x (window at 2f)

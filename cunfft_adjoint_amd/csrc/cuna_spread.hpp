@@ -355,6 +355,9 @@ struct SpreadArgs {
     T bi;       // 1/beta
     T wscale;   // weight prefactor
     T e3[kMaxTaps];  // exp(-o^2/beta), o = olo + q
+    // nullable: per curve (bi, wscale, exp(-bi)) in place of the three above,
+    // for batches of compat curves whose filter widths differ
+    const T *cpar;
     // VAL_COMPAT_BOOT (src/gridding.cu:86-117): value y[d] and weight
     // parameter eps_orig[d] of the drawn point d, position of point p
     uint32_t seed;
@@ -543,6 +546,8 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
     TileRanges R;
     R.init(A, tile, c, NTAPS);
     const int tstart = R.tstart, total = R.total;
+    const T bi = A.cpar ? A.cpar[3 * c] : A.bi;
+    const T wscale = A.cpar ? A.cpar[3 * c + 1] : A.wscale;
 
     for (int g = threadIdx.x; g < TT; g += NT) gacc[g] = T(0);
     for (int base = 0; base < total; base += A.cap) {
@@ -594,7 +599,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
             for (int i = lo; i < hi; ++i) {
                 const StagedPoint<T> sp = pts[i];
                 const T d = (T)(g - sp.a) - sp.e;
-                sum += sp.v * dexp(-d * d * A.bi);
+                sum += sp.v * dexp(-d * d * bi);
             }
             gacc[g] += sum;
         }
@@ -602,7 +607,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
     __syncthreads();
     T *gout = A.grid + (long)slab * A.grid_stride + tstart;
     const int tlen = min(TT, A.G - tstart);
-    for (int g = threadIdx.x; g < tlen; g += NT) gout[g] = gacc[g] * A.wscale;
+    for (int g = threadIdx.x; g < tlen; g += NT) gout[g] = gacc[g] * wscale;
 }
 
 // ---------------------------------------------- fixed-point scatter --
@@ -612,10 +617,12 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
 // not depend on the order of the additions, so the result is bitwise
 // reproducible; float LDS atomics would be neither reproducible nor fast on
 // this hardware, and double LDS atomics are compare-and-swap loops.  A round
-// of up to CAP points uses the scale 2^s with cnt * max|v| * wscale * 2^s <=
-// 2^62, so no cell can overflow, and a weight keeps at least 2^-52 of the
-// round's largest weight as resolution (float itself keeps 2^-24 of the
-// cell's own sum, double 2^-53).
+// of up to CAP points uses the scale 2^s with cnt * wmax * 2^s <= 2^62, wmax
+// the largest weight the round adds to the tile's own cells (bounded by
+// max|v| * wscale for accurate filters), so none of them can overflow (halo
+// cells, which are not written out, may), and a weight keeps at least 2^-52
+// of wmax as resolution (float itself keeps 2^-24 of the cell's own sum,
+// double 2^-53).
 // Split of a scaled weight x (|x| < 2^62) into a two's-complement 64-bit
 // integer: h = floor(x / 2^32) and l = x - h 2^32 in [0, 2^32).  Exact for
 // x >= 0; for -2^31 < x < 0, l is rounded to a multiple of 2^8, an error far
@@ -656,8 +663,12 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
     const int total = R.total;
     const bool multi = total > CAP;
     const int tid = threadIdx.x;
-    const T c1 = A.e3[Q0 + 1];  // exp(-bi): e3[q] = exp(-(q - Q0)^2 bi)
+    // exp(-bi): e3[q] = exp(-(q - Q0)^2 bi)
+    const T c1 = A.cpar ? A.cpar[3 * c + 2] : A.e3[Q0 + 1];
+    const T bi = A.cpar ? A.cpar[3 * c] : A.bi;
+    const T wscale = A.cpar ? A.cpar[3 * c + 1] : A.wscale;
 
+    const int tlen = min(TT, A.G - R.tstart);  // cells of the tile inside the grid
     for (int g = tid; g < NC; g += NT) acc[g] = 0ull;
     if (multi)
         for (int g = tid; g < TT; g += NT) facc[g] = T(0);
@@ -695,10 +706,19 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
             // a non-finite value or position (NaN x gives NaN e) makes the
             // round's maximum infinite, and the tile's output NaN
             T av = fabs(vv[k]);
+            if constexpr (NTAPS % 2 == 1) {
+                // compat filter: e lies up to R_f + 1 cells from the taps and
+                // taps off the grid are dropped, so the largest weight this
+                // tile keeps, at its tap o nearest e, can be far below |v|
+                // (accurate filters always have a tap within half a cell)
+                const int lo = max(-Q0, -am[k]), hi = min(NTAPS - 1 - Q0, tlen - 1 - am[k]);
+                const T o = fmin(fmax(rint(e), T(lo)), T(hi));
+                av = lo <= hi ? av * dexp(-(o - e) * (o - e) * bi) : T(0);
+            }
             if (!(av <= kBig) || (ok && !(fabs(e) <= kBig))) av = T(INFINITY);
             vmax = fmax(vmax, av);
         }
-        // round maximum of |v|
+        // round maximum of |v| (compat: of each point's largest weight / wscale)
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
         if ((tid & 63) == 0) red[tid >> 6] = vmax;
@@ -708,20 +728,20 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
         for (int w = 1; w < NT / 64; ++w) vmax = fmax(vmax, red[w]);
         if (!(vmax <= kBig)) finite = false;
         // 2^sexp: cnt * vmax * wscale * 2^sexp <= 2^62, and wscale * 2^sexp <= 2^kMaxExp
-        const T bound = (T)cnt * vmax * A.wscale;
+        const T bound = (T)cnt * vmax * wscale;
         int eb = 0;
         if (bound > T(0) && finite) (void)frexp(bound, &eb);
         else eb = -64;
         int ew = 0;
-        (void)frexp(A.wscale, &ew);
+        (void)frexp(wscale, &ew);
         sexp = min(62 - eb, kMaxExp - ew);
-        const T mul = ldexp(A.wscale, sexp);
+        const T mul = ldexp(wscale, sexp);
         if (finite) {
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
                 if (base + k * NT + tid < base + cnt) {
                     T w[NTAPS];
-                    tap_weights_rec<T, NTAPS>(vv[k] * mul, ev[k], A.bi, c1, w);
+                    tap_weights_rec<T, NTAPS>(vv[k] * mul, ev[k], bi, c1, w);
                     const int c0 = am[k] - Q0;  // first tap's cell
                     if (R.tiny) {
 #pragma unroll
@@ -753,7 +773,6 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
         }
     }
     T *gout = A.grid + (long)slab * A.grid_stride + R.tstart;
-    const int tlen = min(TT, A.G - R.tstart);
     const T inv = ldexp(T(1), -sexp);
     for (int g = tid; g < tlen; g += NT) {
         const T v = multi ? facc[g] : (T)(long long)acc[HALO + g] * inv;

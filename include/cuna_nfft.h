@@ -154,6 +154,35 @@ int cuna_stream_capture_end(hipStream_t stream, cuna_graph *graph);
 int cuna_graph_launch(cuna_graph graph, hipStream_t stream);
 int cuna_graph_destroy(cuna_graph graph);
 
+/* Batches of curves through the reference formula: the batched transform and
+ * filter generator that the reference keeps commented out
+ * (src/cuna_kernel.cu:302-358 cunfft_adjoint_raw_batched_async,
+ * src/gaussian_filter.cu:67-99 generate_pinned_filter_properties_batch).
+ * Curve i has n[i] points of the concatenated x (and f_data; NULL = unit
+ * weights) and ng[i] grid cells (power of two >= 4, n[i] < 2 ng[i]); its
+ * ng[i] outputs (exactly those of cunfft_adjoint_raw_async on that curve)
+ * go to the start of slab i of f_hat, slab stride idist = max ng, f_hat
+ * holding nlc * idist values; the rest of each slab is left as it was.
+ * n, ng and gpu_fprops are HOST arrays; gpu_fprops[i] is curve i's device
+ * filter_properties from generate_*filter_properties*.  When all of them
+ * are such registered structs, the multi-curve plan (curves of equal ng
+ * share one spread launch and one batched FFT) is cached until one of them
+ * is freed (at most 16 batches; least recently used dropped first); a NULL
+ * array or an unregistered struct rebuilds the plan on every call. */
+void cunfft_adjoint_raw_batched_async(const dTyp *x, const dTyp *f_data, Complex *f_hat,
+                                      const int *n, const int *ng, const int nlc,
+                                      const filter_properties *const *gpu_fprops,
+                                      hipStream_t stream);
+/* generate_pinned_filter_properties for each curve of a batch: h_fprops[i]
+ * (host struct with caller-allocated device E1/E2 of n[i] and E3 of R_f
+ * entries) and d_fprops[i] (device struct) describe curve i, whose points
+ * start at x + n[0] + ... + n[i-1]. */
+void generate_pinned_filter_properties_batch(const dTyp *x, const int *n, const int *ng,
+                                             const int nlc, filter_properties **h_fprops,
+                                             filter_properties **d_fprops, hipStream_t stream);
+/* Frees every cached multi-curve plan. */
+void cuna_batch_cache_clear(void);
+
 /* The FFT stage alone: in-place unnormalised DFT of `batch` sequences of n
  * (power of two) values, data[r] = sum_g data[g] exp(sign*2 pi i g r/n).
  * sign = +1 is the reference's CUFFT_INVERSE. */

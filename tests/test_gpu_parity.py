@@ -4,6 +4,8 @@ Tolerances: compat mode vs the oracle's restatement of the reference formula
 rel L2 <= 1e-5 (libcunaf) / 1e-12 (libcunad); accurate mode vs the FP64 direct
 NDFT rel L2 <= 1e-5 (libcunaf, m = 8) / 1e-12 (libcunad, m = 13).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -146,6 +148,133 @@ def test_compat_unregistered_fprops_rebuilds():
     f_hat = torch.zeros(ng, dtype=torch.complex64, device="cuda")
     cu.cunfft_adjoint_raw(xd, yd, f_hat, n, ng, 0)
     assert rel_l2(host(f_hat), oracle.compat_adjoint(x, y, ng, False)) < 1e-5
+
+
+# ---------------------------------------------- batches of compat curves --
+BATCH_N = [1, 50, 1000, 3000, 7, 200, 5000]
+BATCH_NG = [16, 1024, 4096, 4096, 16, 256, 8192]
+
+
+def _hip_malloc(nbytes):
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    p = ctypes.c_void_p()
+    assert h.hipMalloc(ctypes.byref(p), nbytes) == 0
+    return p.value
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_compat_batched_vs_single_curves(prec):
+    """cunfft_adjoint_raw_batched_async (the reference's commented-out batched
+    API): each curve's slab equals the single-curve transform and the
+    oracle, the rest of each slab is untouched, unit weights work, and the
+    cached multi-curve plan gives the same bits on a second call."""
+    rng = np.random.default_rng(31)
+    n, ng = BATCH_N, BATCH_NG
+    nlc, idist = len(n), max(ng)
+    off = np.concatenate([[0], np.cumsum(n)])
+    x = rng.uniform(-0.5, 0.5, off[-1]).astype(NP[prec])
+    y = rng.standard_normal(off[-1]).astype(NP[prec])
+    xd, yd = dev(x), dev(y)
+    cdt = torch.complex64 if prec == "f" else torch.complex128
+    view = np.uint32 if prec == "f" else np.uint64
+    fps = [cu.generate_filter_properties(xd[off[i]:off[i + 1]], n[i], ng[i]) for i in range(nlc)]
+    s = torch.cuda.Stream()
+    sentinel = 3.0 + 4.0j
+    for f_data, yh in ((yd, y), (None, None)):
+        fh = torch.full((nlc * idist,), sentinel, dtype=cdt, device="cuda")
+        cu.cunfft_adjoint_raw_batched_async(xd, f_data, fh, n, ng, fps, stream=s)
+        s.synchronize()
+        got = host(fh).reshape(nlc, idist)
+        for i in range(nlc):
+            sl = slice(off[i], off[i + 1])
+            single = torch.zeros(ng[i], dtype=cdt, device="cuda")
+            cu.cunfft_adjoint_raw_async(xd[sl], None if f_data is None else yd[sl], single, n[i],
+                                        ng[i], fps[i], stream=s)
+            s.synchronize()
+            assert rel_l2(got[i, :ng[i]], host(single)) < (1e-6 if prec == "f" else 1e-14), i
+            ref = oracle.compat_adjoint(x[sl], None if yh is None else yh[sl], ng[i], prec == "d")
+            assert rel_l2(got[i, :ng[i]], ref) < TOL[prec], i
+            assert np.all(got[i, ng[i]:] == sentinel), i
+    # second call: the cached plan, same bits
+    a = torch.zeros(nlc * idist, dtype=cdt, device="cuda")
+    b = torch.zeros_like(a)
+    cu.cunfft_adjoint_raw_batched_async(xd, yd, a, n, ng, fps, stream=s)
+    cu.cunfft_adjoint_raw_batched_async(xd, yd, b, n, ng, fps, stream=s)
+    s.synchronize()
+    assert np.array_equal(host(a).view(view), host(b).view(view))
+    # freeing one curve's filter properties drops the cached batch; without
+    # filter properties the plan is rebuilt, with the same result
+    fps[0].free()
+    c = torch.zeros_like(a)
+    cu.cunfft_adjoint_raw_batched_async(xd, yd, c, n, ng, None, stream=s)
+    s.synchronize()
+    assert np.array_equal(host(a).view(view), host(c).view(view))
+    for f in fps[1:]:
+        f.free()
+    cu.batch_cache_clear(prec)
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_pinned_filter_properties_batch(prec):
+    """generate_pinned_filter_properties_batch fills each curve's b, normfac,
+    R_f and E1/E2/E3 as src/gaussian_filter.cu:54-57,204-228 define them, and
+    its structs drive the batched transform."""
+    rng = np.random.default_rng(32)
+    n, ng = [300, 20, 4000], [1024, 64, 4096]
+    nlc = len(n)
+    off = np.concatenate([[0], np.cumsum(n)])
+    dt = NP[prec]
+    x = rng.uniform(-0.5, 0.5, off[-1]).astype(dt)
+    y = rng.standard_normal(off[-1]).astype(dt)
+    xd, yd = dev(x), dev(y)
+    L = cu.load(prec)
+    _, _, FP, _ = cu._lib.make_types(prec)
+    rf = L.cuna_filter_radius()
+    tdt = torch.float32 if prec == "f" else torch.float64
+    E1 = [torch.empty(k, dtype=tdt, device="cuda") for k in n]
+    E2 = [torch.empty(k, dtype=tdt, device="cuda") for k in n]
+    E3 = [torch.empty(rf, dtype=tdt, device="cuda") for _ in n]
+    h = [FP(0, 0, 0, E1[i].data_ptr(), E2[i].data_ptr(), E3[i].data_ptr()) for i in range(nlc)]
+    d = [_hip_malloc(ctypes.sizeof(FP)) for _ in range(nlc)]
+    s = torch.cuda.Stream()
+    cu.generate_pinned_filter_properties_batch(xd, n, ng, h, d, stream=s)
+    s.synchronize()
+    rtol = 2e-6 if prec == "f" else 1e-13
+    for i in range(nlc):
+        b, nf, r = oracle.compat_params(n[i], ng[i], prec == "d")
+        assert h[i].filter_radius == r == rf
+        assert h[i].b == pytest.approx(b, rel=1e-7 if prec == "f" else 1e-15)
+        assert h[i].normfac == pytest.approx(nf, rel=1e-7 if prec == "f" else 1e-15)
+        eps = np.array([oracle.compat_point(v, n[i], ng[i], prec == "d")[1]
+                        for v in x[off[i]:off[i + 1]]])
+        np.testing.assert_allclose(host(E1[i]), np.exp(-eps * eps / b) / np.sqrt(np.pi * b),
+                                   rtol=rtol)
+        np.testing.assert_allclose(host(E2[i]), np.exp(2 * eps / b), rtol=rtol)
+        np.testing.assert_allclose(host(E3[i]), np.exp(-np.arange(rf) ** 2 / b), rtol=rtol)
+    idist = max(ng)
+    cdt = torch.complex64 if prec == "f" else torch.complex128
+    fh = torch.zeros(nlc * idist, dtype=cdt, device="cuda")
+    cu.cunfft_adjoint_raw_batched_async(xd, yd, fh, n, ng, d, stream=s)
+    s.synchronize()
+    got = host(fh).reshape(nlc, idist)
+    for i in range(nlc):
+        sl = slice(off[i], off[i + 1])
+        ref = oracle.compat_adjoint(x[sl], y[sl], ng[i], prec == "d")
+        assert rel_l2(got[i, :ng[i]], ref) < TOL[prec], i
+    for p in d:
+        L.free_filter_properties(p, None)
+    cu.batch_cache_clear(prec)
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_compat_batched_rejects_bad_sizes(prec):
+    x = dev(np.zeros(10, dtype=NP[prec]))
+    fh = torch.zeros(64, dtype=torch.complex64 if prec == "f" else torch.complex128,
+                     device="cuda")
+    for n, ng in (([5, 5], [16, 24]), ([5, 5], [16, 2]), ([5, 40], [16, 16]), ([], [])):
+        with pytest.raises(cu.CunaError):
+            cu.cunfft_adjoint_raw_batched_async(x, None, fh, n, ng)
 
 
 # ------------------------------------------------------- accurate mode --
