@@ -14,6 +14,8 @@
 // processed in mirror pairs so that the real post-processing stays local).
 #pragma once
 
+#include <type_traits>
+
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -137,6 +139,53 @@ constexpr int pass_log2(int lg, int p) {
 // 64-byte group.
 __device__ __forceinline__ int lds_sw(int a) { return a ^ ((a >> 4) & 31); }
 
+// ------------------------------------------------------ twiddle tables --
+// Two-level table for exp(+2 pi i e / Nfull), e < Nfull:
+//   lo[e & (B-1)] * hi[e >> log2 B], lo[q] = w^q, hi[q] = w^(q B).
+template <typename T>
+struct Tw2 {
+    const cplx<T> *lo;
+    const cplx<T> *hi;
+    int lb;  // log2 B
+    __device__ __forceinline__ cplx<T> operator()(long e) const {
+        return lo[e & ((1L << lb) - 1)] * hi[e >> lb];
+    }
+    __device__ __forceinline__ cplx<T> operator[](int e) const {
+        return lo[e & ((1 << lb) - 1)] * hi[e >> lb];
+    }
+    // entries of the two tables
+    __host__ __device__ int n_lo() const { return 1 << lb; }
+};
+
+// v[r] *= w^(r k 2^SHIFT), r = 1 .. R-1 (conjugated for S = -1).  From the
+// two-level table (two LDS reads per entry) only w^(k 2^SHIFT) is read; the
+// other powers are products of depth <= log2 R: squarings for the powers of
+// two, and w^r = w^h w^(r-h) with h the largest power of two below r.  From
+// a plain table, or with POWERS off (fewer registers: several butterflies
+// per thread), every power is one lookup.
+template <typename T, int S, int R, int SHIFT, bool POWERS, class TW>
+__device__ __forceinline__ void apply_twiddles(cplx<T> *v, const TW &tw, int k) {
+    if constexpr (POWERS && std::is_same<TW, Tw2<T>>::value) {
+        cplx<T> w[R];
+        w[1] = tw[k << SHIFT];
+        if constexpr (S < 0) w[1].y = -w[1].y;
+#pragma unroll
+        for (int r = 2; r < R; ++r) {
+            const int h = 1 << (31 - __builtin_clz(r));
+            w[r] = h == r ? w[r / 2] * w[r / 2] : w[h] * w[r - h];
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = v[r] * w[r];
+    } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            cplx<T> t = tw[(k * r) << SHIFT];
+            if constexpr (S < 0) t.y = -t.y;
+            v[r] = v[r] * t;
+        }
+    }
+}
+
 // ----------------------------------------------------------- LDS Stockham --
 // W interleaved sequences of length L = 2^LG: element e of sequence w lives at
 // lds[lds_sw(e*W + w)].  Pass 0 reads its inputs through `ld(e, w)`; every pass writes
@@ -184,15 +233,9 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, TW tw, const Loader &ld,
                 // twiddles are global loads: issue them before the barrier
                 // (with several butterflies per thread that would hold all
                 // their twiddles in registers at once; they follow it then)
-                if constexpr (P > 0 && BPT == 1) {
-                    const int k = j & ((1 << LOGNS) - 1);
-#pragma unroll
-                    for (int r = 1; r < R; ++r) {
-                        cplx<T> t = tw[(k * r) << (LG - LOGNS - LR)];
-                        if constexpr (S < 0) t.y = -t.y;
-                        v[q][r] = v[q][r] * t;
-                    }
-                }
+                if constexpr (P > 0 && BPT == 1)
+                    apply_twiddles<T, S, R, LG - LOGNS - LR, true>(v[q], tw,
+                                                                  j & ((1 << LOGNS) - 1));
             }
         }
         if constexpr (P == 0) hook();
@@ -205,14 +248,8 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, TW tw, const Loader &ld,
             if (EXACT || g < NB) {
                 const int w = g % W, j = g / W;
                 const int k = j & ((1 << LOGNS) - 1);
-                if constexpr (P > 0 && BPT > 1) {
-#pragma unroll
-                    for (int r = 1; r < R; ++r) {
-                        cplx<T> t = tw[(k * r) << (LG - LOGNS - LR)];
-                        if constexpr (S < 0) t.y = -t.y;
-                        v[q][r] = v[q][r] * t;
-                    }
-                }
+                if constexpr (P > 0 && BPT > 1)
+                    apply_twiddles<T, S, R, LG - LOGNS - LR, false>(v[q], tw, k);
                 dft<T, R, S>(v[q]);
                 const int base = ((j >> LOGNS) << (LOGNS + LR)) + k;
 #pragma unroll
@@ -270,23 +307,6 @@ __device__ __forceinline__ void lds_fft_inplace(cplx<T> *lds, TW tw) {
     fft_pass<T, S, LG, 1, NT, 0, 0, decltype(fetch), true, NoHook, TW>(lds, tw, fetch);
 }
 
-// ------------------------------------------------------ twiddle tables --
-// Two-level table for exp(+2 pi i e / Nfull), e < Nfull:
-//   lo[e & (B-1)] * hi[e >> log2 B], lo[q] = w^q, hi[q] = w^(q B).
-template <typename T>
-struct Tw2 {
-    const cplx<T> *lo;
-    const cplx<T> *hi;
-    int lb;  // log2 B
-    __device__ __forceinline__ cplx<T> operator()(long e) const {
-        return lo[e & ((1L << lb) - 1)] * hi[e >> lb];
-    }
-    __device__ __forceinline__ cplx<T> operator[](int e) const {
-        return lo[e & ((1 << lb) - 1)] * hi[e >> lb];
-    }
-    // entries of the two tables
-    __host__ __device__ int n_lo() const { return 1 << lb; }
-};
 
 // Device twiddle tables, generated in long double on the host once per size
 // and kept for the life of the process (a few KB each).

@@ -799,8 +799,8 @@ __host__ __device__ inline size_t spread_fx_lds_bytes(int tlog, int ntaps) {
 // which are also order-independent.  The power-of-two scale comes from the
 // curve's max |y| and its densest tap window; an addition that overflows is
 // detected from the returned old value and the item is redone at a smaller
-// scale.  The integer grid occupies the same LDS words as the float grid and
-// is converted in place before the FFT.
+// scale.  The integer grid occupies the same LDS words as the complex
+// buffer of the FFT, whose first pass reads and converts it.
 template <typename T>
 struct BootArgs {
     const PointRec<T> *rec_x;   // (anchor, frac) of x by original index
@@ -842,6 +842,11 @@ __host__ __device__ constexpr size_t boot_lds_bytes(int lg) {
 static __global__ void k_reset_counter(int *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
 }
+
+template <typename I>
+struct alignas(2 * sizeof(I)) FixedPair {
+    I x, y;
+};
 
 template <typename T>
 struct FixedOf;
@@ -990,14 +995,30 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
                 __syncthreads();
                 if (!redo) break;
             }
-            // fixed point -> real, in place
             const T inv = (T)ldexp(1.0, -sexp);
-            for (int i = tid; i < G; i += NT) grid[i] = (T)(I)gu[i] * inv;
+            if constexpr (MODE == 0) {
+                // the FFT's first pass reads the fixed-point grid (complex e
+                // = cells 2e, 2e+1) and converts it to reals as it loads
+                auto fetch = [&](int e, int) __attribute__((always_inline)) {
+                    const FixedPair<I> q = reinterpret_cast<const FixedPair<I> *>(gu)[e];
+                    return cmk((T)q.x * inv, (T)q.y * inv);
+                };
+                fft_pass<T, 1, LG, 1, NT, 0, 0, decltype(fetch), true, NoHook, Tw2<T>>(z, twN,
+                                                                                    fetch);
+            } else {
+                // (Lomb-Scargle keeps the window spectrum in registers; with
+                // the transform outside this branch it needs fewer of them)
+                for (int i = tid; i < G; i += NT) grid[i] = (T)(I)gu[i] * inv;
+            }
         } else {
+            // a non-finite value: every spectrum value is NaN
             for (int i = tid; i < G; i += NT) grid[i] = (T)NAN;
+            if constexpr (MODE == 0) __syncthreads();
         }
-        __syncthreads();
-        lds_fft_inplace<T, 1, LG, NT>(z, twN);
+        if constexpr (MODE == 1) {
+            __syncthreads();
+            lds_fft_inplace<T, 1, LG, NT>(z, twN);
+        }
     };
     // deconvolved spectrum at frequency f (0 <= f < M/2) from z
     auto spectrum = [&](int f) __attribute__((always_inline)) -> cplx<T> {
