@@ -662,7 +662,10 @@ def main(argv=None):
     ap.add_argument("--no-bootstrap", action="store_true")
     ap.add_argument("--curves", type=int, default=1024)
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the precision and differentiation workloads")
+                    help="skip the extra workloads (FP64 and compat variants of the N=1e6 "
+                         "transform, batched compat curves, precision, differentiation)")
+    ap.add_argument("--extras", default="c2_variants,compat_batch,precision,differentiation",
+                    help="comma-separated subset of the extra workloads to run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of exactly 256 MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -724,10 +727,18 @@ def main(argv=None):
         res["bootstrap"] = bench_bootstrap(args, torch, cu)
     if not args.no_extras:
         cu.load("d")
-        res["c2_variants"] = bench_c2_variants(r["x"], r["y"], torch, cu)
-        res["compat_batch"] = bench_compat_batch(torch, cu)
-        res["precision"] = bench_precision(torch, cu)
-        res["differentiation"] = bench_differentiation(torch, cu)
+        extras = {
+            "c2_variants": lambda: bench_c2_variants(r["x"], r["y"], torch, cu),
+            "compat_batch": lambda: bench_compat_batch(torch, cu),
+            "precision": lambda: bench_precision(torch, cu),
+            "differentiation": lambda: bench_differentiation(torch, cu),
+        }
+        for name in args.extras.split(","):
+            if name:
+                if name not in extras:
+                    print(f"unknown extra workload {name!r}", file=sys.stderr)
+                    return 2
+                res[name] = extras[name]()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(r["x"], r["y"], r["M"], args.cpu_seconds)
     print(json.dumps(res))
