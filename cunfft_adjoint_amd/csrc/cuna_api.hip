@@ -369,10 +369,19 @@ void Plan<T>::bootstrap_ls(const T *y, int nbs, uint32_t seed, T *power, T *peak
             "m in {6, 8, 10, 12, 13, 16}",
             CUNA_ERROR_UNSUPPORTED);
     prepare_bootstrap(nbs, true);
+    // window spectra workspace: one row of M/2 values per resident workgroup
+    // (at most one per kMaxLds of LDS on each CU)
+    int dev = 0, ncu = 0;
+    CUNA_HIP(hipGetDevice(&dev));
+    CUNA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int slots = ncu * std::max(1, (int)(kMaxLds / boot_lds_bytes<T>(lgN)));
+    ls_work.ensure(sizeof(cplx<T>) * (size_t)slots * (M / 2));
     BootArgs<T> B = fused_args(y, nbs, seed, s);
     B.ls_power = power;
     B.ls_peak = peak;
     B.ls_kpeak = kpeak;
+    B.ls_w = ls_work.as<cplx<T>>();
+    B.ls_w_slots = slots;
     B.ntr = 1;
     const long total = (long)ncurves() * nbs;
     require(total < (1L << 31), "too many bootstrap items");

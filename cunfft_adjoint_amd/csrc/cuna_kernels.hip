@@ -462,10 +462,7 @@ bool boot_fused_supported(int lgN, int ntaps) {
 
 template <typename T, int LG, int NTAPS, int MODE>
 static void boot_fused_one(BootArgs<T> A, hipStream_t s) {
-    // the Lomb-Scargle mode holds the window's values across the second
-    // transform: at most 512 threads (256 registers each)
-    constexpr int NT0 = fft_threads(LG, 1) < 256 ? 256 : fft_threads(LG, 1);
-    constexpr int NT = MODE == 1 && NT0 > 512 ? 512 : NT0;
+    constexpr int NT = fft_threads(LG, 1) < 256 ? 256 : fft_threads(LG, 1);
     auto &tc = TwiddleCache<T>::get();
     A.twN = tc.two_level(LG);
     A.twG = tc.two_level(LG + 1);
@@ -479,7 +476,12 @@ static void boot_fused_one(BootArgs<T> A, hipStream_t s) {
     int per_cu = 0;
     CUNA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k),
                                                           NT, lds));
-    const int nblk = std::max(1, std::min(A.total, ncu * std::max(per_cu, 1)));
+    int nblk = std::max(1, std::min(A.total, ncu * std::max(per_cu, 1)));
+    if constexpr (MODE == 1) {
+        // one workspace row per workgroup
+        require(A.ls_w != nullptr && A.ls_w_slots >= 1, "Lomb-Scargle workspace missing");
+        nblk = std::min(nblk, A.ls_w_slots);
+    }
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, A.counter);
     CUNA_LAUNCH_CHECK();
     hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, A);

@@ -91,6 +91,7 @@ struct Plan {
     DevMem grid, work, counts, counter;
     // fused bootstrap: per-curve max |y| and densest tap window of x and 2x
     DevMem ymax, cover;
+    DevMem ls_work;  // Lomb-Scargle: window spectra of the resident workgroups
     bool cover_ready = false;
     // optional profiling events, recorded on the execution stream before the
     // first kernel of adjoint() and after each of its kernels

@@ -825,6 +825,10 @@ struct BootArgs {
     T *ls_power;
     T *ls_peak;
     int *ls_kpeak;
+    // ... and a workspace of ls_w_slots rows of M/2 values: a workgroup keeps
+    // its current window spectrum in row blockIdx.x
+    cplx<T> *ls_w;
+    int ls_w_slots;
 };
 
 // LDS of the fused kernel: the grid (G reals = N complex) and the four
@@ -892,11 +896,21 @@ __device__ __forceinline__ T ls_power(cplx<T> F, cplx<T> W, T n, T yy) {
     return yy > T(0) ? p / yy : T(0);
 }
 
+// A value every thread of the workgroup holds equally, moved to scalar
+// registers (it then takes no vector registers across the transforms).
+__device__ __forceinline__ double uniform_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // MODE 0: items (curve, replicate, transform) -> deconvolved spectra.
 // MODE 1: items (curve, replicate) -> Lomb-Scargle power of the replicate:
-// the window transform (unit weights at 2x) is kept in registers for the
-// frequencies this thread emits, then the transform of the mean-subtracted
-// values at x gives the power directly; nothing else is written.
+// the window transform (unit weights at 2x) goes to the workgroup's row of
+// the workspace (each thread writes, and later reads back, its own
+// frequencies), then the transform of the mean-subtracted values at x gives
+// the power directly; nothing else is written.
 template <typename T, int LG, int NTAPS, int NT, int MODE>
 __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
     using I = typename FixedOf<T>::I;
@@ -906,7 +920,6 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
     constexpr int LBN = (LG + 1) / 2, LBG = (LG + 2) / 2;  // as TwiddleCache::two_level
     constexpr int NLN = 1 << LBN, NHN = 1 << (LG - LBN), NLG = 1 << LBG, NHG = 1 << (LG + 1 - LBG);
     constexpr int PB = 4;  // resampled points per thread whose loads are in flight together
-    constexpr int OPT = (N / 2 + NT - 1) / NT;  // Lomb-Scargle frequencies per thread
     constexpr int kMaxExp = sizeof(T) == 4 ? 100 : 1000;
     T *grid = reinterpret_cast<T *>(cuna_smem);
     U *gu = reinterpret_cast<U *>(cuna_smem);
@@ -1031,7 +1044,9 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
     for (;;) {
         if (tid == 0) item_sh = atomicAdd(A.counter, 1);
         __syncthreads();
-        const int item = item_sh;
+        // uniform over the workgroup: scalar registers (so are the curve's
+        // offsets and sizes loaded through it)
+        const int item = __builtin_amdgcn_readfirstlane(item_sh);
         if (item >= A.total) break;  // uniform over the workgroup
         const int per_curve = A.nbs * (MODE == 0 ? A.ntr : 1);
         const int ci = A.order[item / per_curve];
@@ -1096,34 +1111,28 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
                 s1 += red_sh[2 * w];
                 s2 += red_sh[2 * w + 1];
             }
+            s1 = uniform_f64(s1);
+            s2 = uniform_f64(s2);
             const double nd = (double)max(nc, 1);
             const double mean_off = s1 / nd;
             const T ybar = (T)((double)shift + mean_off);
             const T yy = (T)fmax(0.0, s2 - s1 * mean_off);
-            // window W(2w) for this thread's frequencies
+            // window W(2w), parked in this workgroup's workspace row
             transform(c0, nc, rep, fm, A.rec_2x, nullptr, T(0), 1.0, A.cover_2x[ci]);
-            cplx<T> W[OPT];
-#pragma unroll
-            for (int i = 0; i < OPT; ++i) {
-                const int f = tid + i * NT;
-                W[i] = f < half ? spectrum(f) : cmk(T(0), T(0));
-            }
+            cplx<T> *wrow = A.ls_w + (long)blockIdx.x * half;
+            for (int f = tid; f < half; f += NT) wrow[f] = spectrum(f);
             __syncthreads();
             // F(w) of the mean-subtracted values, then the power
             const double vm = (double)A.ymax[ci] + fabs((double)ybar);
             transform(c0, nc, rep, fm, A.rec_x, A.y, ybar, vm, A.cover_x[ci]);
             T best = T(-1);
             int kbest = 0;
-#pragma unroll
-            for (int i = 0; i < OPT; ++i) {
-                const int f = tid + i * NT;
-                if (f < half) {
-                    const T pw = ls_power(spectrum(f), W[i], (T)nc, yy);
-                    if (A.ls_power) A.ls_power[row * half + f] = pw;
-                    if (f >= 1 && pw > best) {
-                        best = pw;
-                        kbest = f;
-                    }
+            for (int f = tid; f < half; f += NT) {
+                const T pw = ls_power(spectrum(f), wrow[f], (T)nc, yy);
+                if (A.ls_power) A.ls_power[row * half + f] = pw;
+                if (f >= 1 && pw > best) {
+                    best = pw;
+                    kbest = f;
                 }
             }
             if (A.ls_peak || A.ls_kpeak) {
