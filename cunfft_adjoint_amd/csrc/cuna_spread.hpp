@@ -683,13 +683,18 @@ __global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
         // Branch-free loads: a slot past the round's end reads the round's
         // first point and is masked afterwards, so that all PPT slots'
         // loads issue together (a branch per slot would wait out each one).
+        // Slot i of the round takes point (i P) mod cnt, P a prime not dividing
+        // cnt (a bijection): the lanes of one wave take points ~P apart in
+        // anchor order rather than neighbours, which often share a cell and
+        // then serialise their atomics.
+        const int P = cnt % 97 ? 97 : 101;
         int am[PPT], pk[PPT];
         T ev[PPT], vv[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
-            const int j = base + k * NT + tid;
+            const int i = k * NT + tid;
             int s, ro;
-            R.get(j < base + cnt ? j : base, s, ro);
+            R.get(base + (i < cnt ? (i * P) % cnt : 0), s, ro);
             const PointRec<T> pr = A.rec[s];
             am[k] = pr.a - ro;
             ev[k] = pr.e;
