@@ -168,6 +168,23 @@ def test_validation_errors_without_gpu(lib):
     out = (R * 8)()
     L.cunfft_adjoint(x, None, out, 4, 6, 0)  # ng = 6: not a power of two
     assert L.cuna_get_last_error() != 0
+    # batched compat API: sizes are checked before any device work
+    fake = ctypes.c_void_p(256)
+    n2, ng2 = (ctypes.c_int * 2)(5, 5), (ctypes.c_int * 2)(16, 16)
+    L.cunfft_adjoint_raw_batched_async(fake, None, None, n2, ng2, 2, None, None)
+    assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE  # f_hat NULL
+    L.cunfft_adjoint_raw_batched_async(fake, None, fake, n2, ng2, 0, None, None)
+    assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE  # no curves
+    L.cunfft_adjoint_raw_batched_async(fake, None, fake, n2, (ctypes.c_int * 2)(16, 24), 2,
+                                       None, None)
+    assert L.cuna_get_last_error() == _lib.CUNA_ERROR_UNSUPPORTED  # ng = 24
+    L.cunfft_adjoint_raw_batched_async(fake, None, fake, (ctypes.c_int * 2)(5, 40), ng2, 2,
+                                       None, None)
+    assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE  # n >= 2 ng
+    assert b"2*ng" in L.cuna_get_last_error_string()
+    L.generate_pinned_filter_properties_batch(fake, n2, ng2, 2, None, None, None)
+    assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE
+    L.cuna_clear_last_error()
 
 
 def test_abort_on_error_is_opt_in(tmp_path):
