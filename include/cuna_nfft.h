@@ -168,7 +168,10 @@ int cuna_graph_destroy(cuna_graph graph);
  * are such registered structs, the multi-curve plan (curves of equal ng
  * share one spread launch and one batched FFT) is cached until one of them
  * is freed (at most 16 batches; least recently used dropped first); a NULL
- * array or an unregistered struct rebuilds the plan on every call. */
+ * array or an unregistered struct rebuilds the plan on every call.  As with
+ * the reference's E1/E2 arrays, the cached plan belongs to the x it was made
+ * from: regenerate the filter properties when x changes.  Like a plan, a
+ * cached batch executes on one stream at a time. */
 void cunfft_adjoint_raw_batched_async(const dTyp *x, const dTyp *f_data, Complex *f_hat,
                                       const int *n, const int *ng, const int nlc,
                                       const filter_properties *const *gpu_fprops,
