@@ -383,7 +383,7 @@ static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s) {
         require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
         auto k = k_spread_fx<T, NTAPS, VM>;
         set_lds(k, lds);
-        hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
+        hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(kSpreadFxThreads), lds, s, A);
         CUNA_LAUNCH_CHECK();
         return;
     }

@@ -640,9 +640,13 @@ __device__ __forceinline__ unsigned long long to_fixed64(double x) {
     return ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)l;
 }
 
+// threads per workgroup and point slots per thread of the fixed-point scatter
+constexpr int kSpreadFxThreads = 256, kSpreadFxSlots = 4;
+
 template <typename T, int NTAPS, int VM>
-__global__ __launch_bounds__(256) void k_spread_fx(SpreadArgs<T> A) {
-    constexpr int NT = 256, PPT = 4, CAP = NT * PPT, Q0 = (NTAPS - 1) / 2, HALO = NTAPS;
+__global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A) {
+    constexpr int NT = kSpreadFxThreads, PPT = kSpreadFxSlots, CAP = NT * PPT;
+    constexpr int Q0 = (NTAPS - 1) / 2, HALO = NTAPS;
     const int TT = 1 << A.tlog;
     const int NC = TT + 2 * HALO;  // cells -HALO .. TT+HALO-1
     unsigned long long *acc = reinterpret_cast<unsigned long long *>(cuna_smem);
