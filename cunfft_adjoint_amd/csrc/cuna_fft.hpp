@@ -249,7 +249,7 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, TW tw, const Loader &ld,
                 const int w = g % W, j = g / W;
                 const int k = j & ((1 << LOGNS) - 1);
                 if constexpr (P > 0 && BPT > 1)
-                    apply_twiddles<T, S, R, LG - LOGNS - LR, false>(v[q], tw, k);
+                    apply_twiddles<T, S, R, LG - LOGNS - LR, (R <= 4)>(v[q], tw, k);
                 dft<T, R, S>(v[q]);
                 const int base = ((j >> LOGNS) << (LOGNS + LR)) + k;
 #pragma unroll
