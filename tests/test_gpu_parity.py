@@ -268,6 +268,43 @@ def test_pinned_filter_properties_batch(prec):
 
 
 @pytest.mark.parametrize("prec", ["f", "d"])
+def test_pinned_filter_properties_single(prec):
+    """generate_pinned_filter_properties (inc/cuna_filter.h:37-40): the caller
+    owns the host struct, its device E1/E2/E3 arrays and the device struct;
+    the call fills them on a stream (src/gaussian_filter.cu:43-66), and the
+    raw transform then matches the oracle."""
+    rng = np.random.default_rng(33)
+    n, ng = 2500, 1 << 12
+    x = rng.uniform(-0.5, 0.5, n).astype(NP[prec])
+    y = rng.standard_normal(n).astype(NP[prec])
+    xd, yd = dev(x), dev(y)
+    L = cu.load(prec)
+    _, _, FP, _ = cu._lib.make_types(prec)
+    rf = L.cuna_filter_radius()
+    tdt = torch.float32 if prec == "f" else torch.float64
+    E1, E2, E3 = (torch.empty(k, dtype=tdt, device="cuda") for k in (n, n, rf))
+    h = FP(0, 0, 0, E1.data_ptr(), E2.data_ptr(), E3.data_ptr())
+    d = _hip_malloc(ctypes.sizeof(FP))
+    s = torch.cuda.Stream()
+    cu.generate_pinned_filter_properties(xd, n, ng, h, d, stream=s)
+    s.synchronize()
+    b, nf, r = oracle.compat_params(n, ng, prec == "d")
+    assert h.filter_radius == r == rf
+    assert h.b == pytest.approx(b, rel=1e-7 if prec == "f" else 1e-15)
+    eps = np.array([oracle.compat_point(v, n, ng, prec == "d")[1] for v in x[:300]])
+    rtol = 2e-6 if prec == "f" else 1e-13
+    np.testing.assert_allclose(host(E1)[:300], np.exp(-eps * eps / b) / np.sqrt(np.pi * b),
+                               rtol=rtol)
+    np.testing.assert_allclose(host(E3), np.exp(-np.arange(rf) ** 2 / b), rtol=rtol)
+    fh = torch.zeros(ng, dtype=torch.complex64 if prec == "f" else torch.complex128,
+                     device="cuda")
+    cu.cunfft_adjoint_raw_async(xd, yd, fh, n, ng, d, stream=s)
+    s.synchronize()
+    assert rel_l2(host(fh), oracle.compat_adjoint(x, y, ng, prec == "d")) < TOL[prec]
+    L.free_filter_properties(d, None)
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
 def test_compat_batched_rejects_bad_sizes(prec):
     x = dev(np.zeros(10, dtype=NP[prec]))
     fh = torch.zeros(64, dtype=torch.complex64 if prec == "f" else torch.complex128,
