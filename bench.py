@@ -672,8 +672,9 @@ def main(argv=None):
     ap.add_argument("--boot-steps", type=int, default=4)
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build measures the single-GPU kernel; "
-              "how the transform would shard over several GPUs is described in DESIGN.md",
+        print("bench.py runs on one GPU only: it measures the single-GPU kernels. Over several "
+              "GPUs a transform would shard by points (one reduce-scatter of the spectra) and "
+              "batches by curve (no exchange); that path is not built",
               file=sys.stderr)
         return 2
 

@@ -24,7 +24,8 @@ BUILDDIR = os.path.join(HERE, "_build")
 SOURCES = ["cuna_kernels.hip", "cuna_api.hip", "cuna_tune.hip"]
 ARCH = os.environ.get("CUNA_OFFLOAD_ARCH", "gfx950")
 
-# packed-fp32 instructions are disabled for all device code
+# packed-fp32 instructions (v_pk_*_f32) are disabled for all device code: some
+# of their forms have given wrong results on this hardware without an error
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -243,11 +243,12 @@ def test_compat_structure():
             lo, hi = max(0, j - R + 1), min(ng - 1, j + R - 1)
             assert nz.min() >= lo and nz.max() <= hi
             # the one-sided window: the largest tap sits at the top end
-            # (eps in [R_f - 1, R_f + 1), App. B-2/B-3)
+            # (eps in [R_f - 1, R_f + 1): u truncates ng*x - R_f toward zero,
+            # src/gaussian_filter.cu:213-214)
             u, eps, jj = oracle.compat_point(x0, 2, ng, dbl)
             assert jj == j and R - 1 <= eps < R + 1
-    # linearity in y and the survey's measured bias vs the NDFT (App. C:
-    # rel L2 1.31-1.46 at n=1e4, ng=2^15)
+    # linearity in y, and the reference formula's distance from the NDFT:
+    # rel L2 between 1.2 and 1.6 at n=1e4, ng=2^15 (its window is one-sided)
     x, y = uniform_points(10000, 11, np.float32)
     y2 = np.random.default_rng(3).standard_normal(10000).astype(np.float32)
     a = oracle.compat_adjoint(x, y, 1 << 15, False)
