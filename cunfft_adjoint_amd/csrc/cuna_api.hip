@@ -720,7 +720,7 @@ static std::shared_ptr<Plan<R>> compat_plan_for(const R *x, int n, int ng,
     return std::shared_ptr<Plan<R>>(std::move(fresh));
 }
 
-// the reference accepts any ng for cuFFT; this build's FFT takes powers of two
+// the reference accepts any ng for cuFFT; this library's FFT takes powers of two
 static void check_compat_sizes(int n, int ng) {
     require(n > 0 && ng > 0, "n and ng must be positive");
     require(ng >= 4 && is_pow2(ng), "ng must be a power of two >= 4", CUNA_ERROR_UNSUPPORTED);

@@ -43,7 +43,7 @@ void generate_filter_properties(const dTyp *x, const int n, const int ng,
  * the device struct, the host struct and the associated spread plan. */
 void free_filter_properties(filter_properties *d_fp, filter_properties *fp);
 
-/* FILTER_RADIUS of this build (6 in libcunaf, 12 in libcunad:
+/* FILTER_RADIUS of this library (6 in libcunaf, 12 in libcunad:
  * src/gaussian_filter.cu:29-33).  Not exported by the reference, which
  * nevertheless requires callers of generate_pinned_filter_properties to
  * allocate E3 with that many values. */
