@@ -199,8 +199,10 @@ def bench_c2(args, torch, cu):
     total_ms, enqueue_ms = held(lambda k: plan.adjoint(yd, out, stream=stream))
     ms_step = total_ms / K
     host_bound = enqueue_ms > hold_ms
-    # a second pass with events around each kernel (per-kernel durations for
-    # the roofline; the events add small gaps, so this pass is not `value`)
+    # a second pass with profiling events (per-kernel durations for the
+    # roofline): the spread's start and stop are taken by its own dispatch;
+    # the FFT passes are bracketed by event markers, which add small gaps, so
+    # this pass is not `value`
     stage = ev.create(4 * K)
 
     def staged(k):

@@ -274,9 +274,8 @@ void Plan<T>::adjoint(const T *y, cplx<T> *out, hipStream_t s) {
     A.y = y;
     A.grid = grid.as<T>();
     A.grid_stride = G;
-    if (prof[0]) CUNA_HIP(hipEventRecord(prof[0], s));
-    launch_spread<T>(A, ntaps, VAL_PLAIN, nc, s);
-    if (prof[1]) CUNA_HIP(hipEventRecord(prof[1], s));
+    // prof[0] and prof[1] take the spread kernel's own start and stop times
+    launch_spread<T>(A, ntaps, VAL_PLAIN, nc, s, prof[0], prof[1]);
     run_fft(grid.as<T>(), out, nc, s, prof[2]);
     if (prof[3]) CUNA_HIP(hipEventRecord(prof[3], s));
 }

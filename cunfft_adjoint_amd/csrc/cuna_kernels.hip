@@ -1,5 +1,7 @@
 // cuna_kernels.hip -- instantiation and launch of the FFT and spread kernels
 // for the library's real type (float in libcunaf.so, double in libcunad.so).
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cstdlib>
 
@@ -376,23 +378,34 @@ void launch_buckets(const SortWork &W, int K, int *bucket, hipStream_t s) {
 }
 
 // --------------------------------------------------------------- spread --
+// with ev0/ev1 set, the kernel's own start and stop times are recorded in
+// them (as its dispatch begins and ends, without separate event packets)
+template <typename K, typename T>
+static void launch_timed(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s, hipEvent_t ev0,
+                         hipEvent_t ev1, const SpreadArgs<T> &A) {
+    if (ev0 || ev1)
+        hipExtLaunchKernelGGL(k, grid, block, (uint32_t)lds, s, ev0, ev1, 0, A);
+    else
+        hipLaunchKernelGGL(k, grid, block, lds, s, A);
+    CUNA_LAUNCH_CHECK();
+}
+
 template <typename T, int NTAPS, int VM>
-static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s) {
+static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s, hipEvent_t ev0,
+                       hipEvent_t ev1) {
     if (!A.gather) {
         const size_t lds = spread_fx_lds_bytes<T>(A.tlog, NTAPS);
         require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
         auto k = k_spread_fx<T, NTAPS, VM>;
         set_lds(k, lds);
-        hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(kSpreadFxThreads), lds, s, A);
-        CUNA_LAUNCH_CHECK();
+        launch_timed(k, dim3(A.ntiles, nslabs), dim3(kSpreadFxThreads), lds, s, ev0, ev1, A);
         return;
     }
     const size_t lds = spread_lds_bytes<T>(A.tlog, NTAPS, A.cap);
     require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
     auto k = k_spread<T, NTAPS, VM>;
     set_lds(k, lds);
-    hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
-    CUNA_LAUNCH_CHECK();
+    launch_timed(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, ev0, ev1, A);
 }
 
 // accurate filters have 2m taps, m in [2, kMaxTaps/2]; compat filters have
@@ -411,19 +424,20 @@ static void dispatch_m(int m, F &&f) {
 }
 
 template <typename T>
-void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStream_t s) {
+void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStream_t s,
+                   hipEvent_t ev0, hipEvent_t ev1) {
     if (ntaps == 11 || ntaps == 23) {
         require(vm != VAL_COUNTS, "counts spread needs an accurate filter");
         if (ntaps == 11) {
             if (vm == VAL_PLAIN)
-                spread_one<T, 11, VAL_PLAIN>(A, nslabs, s);
+                spread_one<T, 11, VAL_PLAIN>(A, nslabs, s, ev0, ev1);
             else
-                spread_one<T, 11, VAL_COMPAT_BOOT>(A, nslabs, s);
+                spread_one<T, 11, VAL_COMPAT_BOOT>(A, nslabs, s, ev0, ev1);
         } else {
             if (vm == VAL_PLAIN)
-                spread_one<T, 23, VAL_PLAIN>(A, nslabs, s);
+                spread_one<T, 23, VAL_PLAIN>(A, nslabs, s, ev0, ev1);
             else
-                spread_one<T, 23, VAL_COMPAT_BOOT>(A, nslabs, s);
+                spread_one<T, 23, VAL_COMPAT_BOOT>(A, nslabs, s, ev0, ev1);
         }
         return;
     }
@@ -432,9 +446,9 @@ void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStr
     dispatch_m<2, kMaxTaps / 2>(ntaps / 2, [&](auto c) {
         constexpr int NTP = decltype(c)::value;
         if (vm == VAL_PLAIN)
-            spread_one<T, NTP, VAL_PLAIN>(A, nslabs, s);
+            spread_one<T, NTP, VAL_PLAIN>(A, nslabs, s, ev0, ev1);
         else
-            spread_one<T, NTP, VAL_COUNTS>(A, nslabs, s);
+            spread_one<T, NTP, VAL_COUNTS>(A, nslabs, s, ev0, ev1);
     });
 }
 
@@ -544,7 +558,8 @@ template void launch_sort_pass<dTyp>(const SortWork &, int, const int *, const P
                                      const int *, PointRec<dTyp> *, int *, hipStream_t);
 template void launch_tile_keys<dTyp>(const PointRec<dTyp> *, long, int, int, int, int, int *,
                                      hipStream_t);
-template void launch_spread<dTyp>(const SpreadArgs<dTyp> &, int, int, int, hipStream_t);
+template void launch_spread<dTyp>(const SpreadArgs<dTyp> &, int, int, int, hipStream_t, hipEvent_t,
+                                  hipEvent_t);
 template void launch_gather<dTyp>(const GatherArgs<dTyp> &, int, int, hipStream_t);
 template bool boot_fused_supported<dTyp>(int, int);
 template void launch_boot_fused<dTyp>(BootArgs<dTyp>, int, int, int, hipStream_t);

@@ -68,8 +68,10 @@ void launch_tile_keys(const PointRec<T> *rec, long n, int tlog, int olo, int ohi
 // bucket offsets of the last sort pass (over K keys)
 void launch_buckets(const SortWork &W, int K, int *bucket, hipStream_t s);
 
+// ev0/ev1 (optional) receive the spread kernel's own start and stop times
 template <typename T>
-void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStream_t s);
+void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStream_t s,
+                   hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 template <typename T>
 void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s);
 

@@ -125,10 +125,11 @@ int cuna_nfft_backward(cuna_nfft_plan plan, const dTyp *y, const Complex *grad_o
 
 int cuna_nfft_plan_info(cuna_nfft_plan plan, cuna_nfft_info *info);
 
-/* Profiling: with events set, cuna_nfft_adjoint records events[0] before its
- * first kernel, events[1] after the spread, events[2] between the two FFT
- * passes (large grids only) and events[3] after the last kernel, all on the
- * execution stream.  nevents = 0 clears them. */
+/* Profiling: with events set, cuna_nfft_adjoint records in events[0] and
+ * events[1] the start and stop times of its spread kernel (taken by that
+ * kernel's dispatch), events[2] between the two FFT passes (large grids
+ * only) and events[3] after the last kernel, all on the execution stream.
+ * nevents = 0 clears them. */
 int cuna_nfft_set_profile_events(cuna_nfft_plan plan, hipEvent_t *events, int nevents);
 int cuna_nfft_plan_destroy(cuna_nfft_plan plan);
 
