@@ -434,6 +434,44 @@ def test_nonfinite_value_propagates(prec):
         assert np.isnan(out).all()
 
 
+def test_profile_events_ordered_and_harmless():
+    """cuna_nfft_set_profile_events: the spread's start and stop (taken by its
+    own dispatch), the mark between the FFT passes and the end come back in
+    order with a positive spread time, and profiling does not change the
+    result (bitwise)."""
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                      ctypes.c_void_p]
+    h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    x, y = clustered_points(200_000, 41)
+    M = 1 << 18  # two FFT passes
+    plan = cu.NFFTPlan(dev(x), M)
+    yd = dev(y)
+    ref = host(plan.adjoint(yd))
+    ev = []
+    for _ in range(4):
+        e = ctypes.c_void_p()
+        assert h.hipEventCreate(ctypes.byref(e)) == 0
+        ev.append(e.value)
+    s = torch.cuda.Stream()
+    arr = (ctypes.c_void_p * 4)(*ev)
+    assert plan.L.cuna_nfft_set_profile_events(plan.h, arr, 4) == 0
+    out = plan.adjoint(yd, stream=s)
+    s.synchronize()
+    assert plan.L.cuna_nfft_set_profile_events(plan.h, None, 0) == 0
+    got = host(out)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    t = ctypes.c_float()
+    ms = []
+    for a, b in ((0, 1), (1, 2), (2, 3)):
+        assert h.hipEventElapsedTime(ctypes.byref(t), ev[a], ev[b]) == 0
+        ms.append(t.value)
+    assert ms[0] > 0 and ms[1] >= 0 and ms[2] >= 0, ms
+    for e in ev:
+        h.hipEventDestroy(e)
+
+
 @pytest.mark.parametrize("prec", ["f", "d"])
 def test_mostly_empty_tiles_repeat_bitwise(prec):
     """Points in 3% of the grid: most spread tiles are empty.  An empty tile
