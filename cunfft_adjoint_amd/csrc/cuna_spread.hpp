@@ -892,17 +892,26 @@ __device__ __forceinline__ cplx<T> real_spectrum_at(cplx<T> Zr, cplx<T> Znr, cpl
 //   tan 2wtau = Im W / Re W,  YC = Re F cos wtau + Im F sin wtau,
 //   YS = Im F cos wtau - Re F sin wtau,  CC = (n + |W|)/2,  SS = (n - |W|)/2,
 //   P = (YC^2/CC + YS^2/SS) / yy   (a term whose denominator vanishes is 0).
+// Reciprocals: the FP32 library uses the hardware's (v_rcp_f32, v_rsq_f32,
+// about 1 ulp; the periodogram's error budget is far larger), FP64 divides.
+__device__ __forceinline__ float ls_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double ls_rcp(double v) { return 1.0 / v; }
+__device__ __forceinline__ float ls_rsqrt(float v) { return __builtin_amdgcn_rsqf(v); }
+__device__ __forceinline__ double ls_rsqrt(double v) { return 1.0 / sqrt(v); }
+
 template <typename T>
 __device__ __forceinline__ T ls_power(cplx<T> F, cplx<T> W, T n, T yy) {
-    const T h = sqrt(W.x * W.x + W.y * W.y);
-    const T c2 = h > T(0) ? W.x / h : T(1), s2 = h > T(0) ? W.y / h : T(0);
+    const T h2 = W.x * W.x + W.y * W.y;
+    const T ih = h2 > T(0) ? ls_rsqrt(h2) : T(0);
+    const T h = h2 * ih;
+    const T c2 = h2 > T(0) ? W.x * ih : T(1), s2 = h2 > T(0) ? W.y * ih : T(0);
     const T cw = sqrt(fmax(T(0), (T(1) + c2) * T(0.5)));
     const T sw = copysign(sqrt(fmax(T(0), (T(1) - c2) * T(0.5))), s2);
     const T yc = F.x * cw + F.y * sw, ys = F.y * cw - F.x * sw;
     const T cc = (n + h) * T(0.5), ss = (n - h) * T(0.5);
     const T tiny = n * T(sizeof(T) == 4 ? 1e-6 : 1e-14);
-    T p = (cc > tiny ? yc * yc / cc : T(0)) + (ss > tiny ? ys * ys / ss : T(0));
-    return yy > T(0) ? p / yy : T(0);
+    T p = (cc > tiny ? yc * yc * ls_rcp(cc) : T(0)) + (ss > tiny ? ys * ys * ls_rcp(ss) : T(0));
+    return yy > T(0) ? p * ls_rcp(yy) : T(0);
 }
 
 // A value every thread of the workgroup holds equally, moved to scalar
