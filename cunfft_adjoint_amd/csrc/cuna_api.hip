@@ -192,6 +192,32 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
     } else {
         launch_buckets(W, Kc, bucket.as<int>(), s);
     }
+    // the spread's per-tile cover (bounds the sums any cell receives)
+    {
+        SpreadArgs<T> C{};
+        C.rec = rec.as<PointRec<T>>();
+        C.bucket = bucket.as<int>();
+        C.coff = coff.as<int>();
+        C.G = G;
+        C.tlog = tlog;
+        C.ntiles = ntiles;
+        C.Kc = Kc;
+        C.olo = olo;
+        C.ntaps = ntaps;
+        C.periodic = periodic ? 1 : 0;
+        C.wrap1 = (periodic && ntiles == 1) ? 1 : 0;
+        const size_t ncov = (size_t)std::max(ncurves, 1) * std::max(ntiles, 1);
+        tile_cover.alloc(sizeof(int) * ncov);
+        launch_tile_cover<T>(C, ncurves, tile_cover.as<int>(), s);
+        // the largest cover picks the spread's accumulator (read back once)
+        std::vector<int> hc(ncov, 0);
+        if (ncurves > 0) {
+            CUNA_HIP(hipMemcpyAsync(hc.data(), tile_cover.ptr, sizeof(int) * ncov,
+                                    hipMemcpyDeviceToHost, s));
+            CUNA_HIP(hipStreamSynchronize(s));
+        }
+        max_cover = *std::max_element(hc.begin(), hc.end());
+    }
     // work-queue order of the curves for the fused batched kernels
     std::vector<int> ord(ncurves);
     for (int c = 0; c < ncurves; ++c) ord[c] = c;
@@ -230,6 +256,8 @@ SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
     A.bi = T(1) / beta;
     A.wscale = wscale;
     A.cpar = cpar_c;
+    A.tile_cover = P.tile_cover.template as<int>();
+    A.fixed32 = (sizeof(T) == 4 && P.max_cover <= kCover32) ? 1 : 0;
     for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
     return A;
 }

@@ -396,7 +396,9 @@ static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s, hipEve
     if (!A.gather) {
         const size_t lds = spread_fx_lds_bytes<T>(A.tlog, NTAPS);
         require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
-        auto k = k_spread_fx<T, NTAPS, VM>;
+        auto k = k_spread_fx<T, NTAPS, VM, false>;
+        if constexpr (sizeof(T) == 4)
+            if (A.fixed32) k = k_spread_fx<T, NTAPS, VM, true>;
         set_lds(k, lds);
         launch_timed(k, dim3(A.ntiles, nslabs), dim3(kSpreadFxThreads), lds, s, ev0, ev1, A);
         return;
@@ -450,6 +452,16 @@ void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStr
         else
             spread_one<T, NTP, VAL_COUNTS>(A, nslabs, s, ev0, ev1);
     });
+}
+
+template <typename T>
+void launch_tile_cover(const SpreadArgs<T> &A, int ncurves, int *cover, hipStream_t s) {
+    if (ncurves <= 0 || A.ntiles <= 0) return;
+    const size_t lds = sizeof(int) * (((size_t)1 << A.tlog) + 2 * (size_t)A.ntaps);
+    require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
+    set_lds(k_tile_cover<T>, lds);
+    hipLaunchKernelGGL(k_tile_cover<T>, dim3(A.ntiles, ncurves), dim3(256), lds, s, A, cover);
+    CUNA_LAUNCH_CHECK();
 }
 
 template <typename T>
@@ -561,6 +573,7 @@ template void launch_tile_keys<dTyp>(const PointRec<dTyp> *, long, int, int, int
 template void launch_spread<dTyp>(const SpreadArgs<dTyp> &, int, int, int, hipStream_t, hipEvent_t,
                                   hipEvent_t);
 template void launch_gather<dTyp>(const GatherArgs<dTyp> &, int, int, hipStream_t);
+template void launch_tile_cover<dTyp>(const SpreadArgs<dTyp> &, int, int *, hipStream_t);
 template bool boot_fused_supported<dTyp>(int, int);
 template void launch_boot_fused<dTyp>(BootArgs<dTyp>, int, int, int, hipStream_t);
 template void launch_curve_cover<dTyp>(const PointRec<dTyp> *, const int *, int, int, int, int *,

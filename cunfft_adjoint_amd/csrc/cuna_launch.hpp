@@ -74,6 +74,9 @@ void launch_spread(const SpreadArgs<T> &A, int ntaps, int vm, int nslabs, hipStr
                    hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 template <typename T>
 void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s);
+// per (curve, tile) point cover of the spread's scale (A: rec, bucket, tiles)
+template <typename T>
+void launch_tile_cover(const SpreadArgs<T> &A, int ncurves, int *cover, hipStream_t s);
 
 // fused batched bootstrap (grid in LDS): false when this size/filter has no
 // fused kernel

@@ -358,6 +358,10 @@ struct SpreadArgs {
     // nullable: per curve (bi, wscale, exp(-bi)) in place of the three above,
     // for batches of compat curves whose filter widths differ
     const T *cpar;
+    // nullable: per (curve, tile), the most points whose anchors fall in one
+    // window of ntaps consecutive anchors (no cell receives more weights)
+    const int *tile_cover;
+    int fixed32;  // FP32 scatter: every tile's cover is at most kCover32
     // VAL_COMPAT_BOOT (src/gridding.cu:86-117): value y[d] and weight
     // parameter eps_orig[d] of the drawn point d, position of point p
     uint32_t seed;
@@ -613,45 +617,105 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
 // ---------------------------------------------- fixed-point scatter --
 // Scatter form (both libraries): a thread takes a point, forms its NTAPS
 // weights with three exponentials, and adds them to the tile's LDS cells
-// with 64-bit integer atomics (ds_add_u64) in fixed point.  Integer sums do
-// not depend on the order of the additions, so the result is bitwise
-// reproducible; float LDS atomics would be neither reproducible nor fast on
-// this hardware, and double LDS atomics are compare-and-swap loops.  A round
-// of up to CAP points uses the scale 2^s with cnt * wmax * 2^s <= 2^62, wmax
-// the largest weight the round adds to the tile's own cells (bounded by
-// max|v| * wscale for accurate filters), so none of them can overflow (halo
-// cells, which are not written out, may), and a weight keeps at least 2^-52
-// of wmax as resolution (float itself keeps 2^-24 of the cell's own sum,
-// double 2^-53).
+// with integer atomics in fixed point (32-bit cells in FP32, 64-bit in
+// FP64).  Integer sums do not depend on the order of the additions, so the
+// result is bitwise reproducible; float LDS atomics would be neither
+// reproducible nor fast on this hardware, and double LDS atomics are
+// compare-and-swap loops.  A round of up to CAP points uses the scale 2^s
+// with min(cnt, cover) * wmax * 2^s <= 2^headroom (cover: the most points of
+// the tile in one window of ntaps anchors), wmax the largest weight the round adds
+// to the tile's own cells (bounded by max|v| * wscale for accurate filters),
+// so none of them can overflow (halo cells, which are not written out, may).
 // Split of a scaled weight x (|x| < 2^62) into a two's-complement 64-bit
-// integer: h = floor(x / 2^32) and l = x - h 2^32 in [0, 2^32).  Exact for
-// x >= 0; for -2^31 < x < 0, l is rounded to a multiple of 2^8, an error far
-// below the float weight's own rounding.
+// integer: h = floor(x / 2^32) and l = x - h 2^32 in [0, 2^32).  In float,
+// exact for x >= 0; for -2^31 < x < 0, l is rounded to a multiple of 2^8,
+// an error far below the float weight's own rounding.  In double, l is exact
+// and rounded to the nearest integer (at most 2^32 - 1, so a tiny negative x
+// gives -1, one unit of the scale).
 __device__ __forceinline__ unsigned long long to_fixed64(float x) {
     const float h = floorf(x * 0x1p-32f);
     const float l = fminf(fmaf(h, -0x1p32f, x), 4294967040.0f);
     return ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)l;
 }
-// The same in double: l is exact, and is rounded to the nearest integer
-// (at most 2^32 - 1, so a tiny negative x gives -1, one unit of the scale).
 __device__ __forceinline__ unsigned long long to_fixed64(double x) {
     const double h = floor(x * 0x1p-32);
     const double l = fmin(rint(fma(h, -0x1p32, x)), 4294967295.0);
     return ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)l;
 }
 
+// Plan time, one workgroup per (tile, curve): cover[c * ntiles + tile] = the
+// most points (the tile's own and its neighbours' edge classes) whose anchors
+// fall in one window of ntaps consecutive anchors that reaches a cell of the
+// tile, from a histogram of the anchors in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tile_cover(SpreadArgs<T> A, int *cover) {
+    constexpr int NT = 256;
+    int *hist = reinterpret_cast<int *>(cuna_smem);
+    __shared__ int red[NT / 64];
+    const int tile = blockIdx.x, c = blockIdx.y;
+    const int TT = 1 << A.tlog, nt = A.ntaps, Q0 = (nt - 1) / 2, NH = TT + 2 * nt;
+    TileRanges R;
+    R.init(A, tile, c, nt);
+    for (int i = threadIdx.x; i < NH; i += NT) hist[i] = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < R.total; j += NT) {
+        int s, ro;
+        R.get(j, s, ro);
+        const int a = A.rec[s].a - ro + nt;
+        if (a >= 0 && a < NH) atomicAdd(&hist[a], 1);
+    }
+    __syncthreads();
+    // cell g takes the points with anchors g - (nt - 1 - Q0) .. g + Q0
+    int best = 0;
+    for (int g = threadIdx.x; g < TT; g += NT) {
+        int sum = 0;
+        for (int q = 0; q < nt; ++q) sum += hist[g - (nt - 1 - Q0) + nt + q];
+        best = max(best, sum);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = red[0];
+        for (int w = 1; w < NT / 64; ++w) m = max(m, red[w]);
+        cover[(long)c * A.ntiles + tile] = m;
+    }
+}
+
+// Accumulators of the scatter.  32-bit cells (ds_add_u32, faster than the
+// 64-bit form, and one conversion instruction per weight) hold a round's
+// sums within 2^30: a weight keeps 2^-30 * min(cnt, cover) of the round's
+// largest weight as resolution, which is the float's own 2^-24 or better for
+// a tile whose cover is at most kCover32 points.  Denser tiles, and the FP64
+// library, use 64-bit cells within 2^62.
+constexpr int kCover32 = 64;
+template <typename T>
+struct Fixed64 {
+    using U = unsigned long long;
+    static constexpr int headroom = 62;
+    __device__ static U fixed(T x) { return to_fixed64(x); }
+};
+struct Fixed32 {
+    using U = unsigned;
+    static constexpr int headroom = 30;
+    __device__ static U fixed(float x) { return (U)__float2int_rn(x); }
+};
+
 // threads per workgroup and point slots per thread of the fixed-point scatter
 constexpr int kSpreadFxThreads = 256, kSpreadFxSlots = 4;
 
-template <typename T, int NTAPS, int VM>
-__global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A) {
+template <typename T, int NTAPS, int VM, class FX>
+__device__ __forceinline__ void spread_fx_body(const SpreadArgs<T> &A, T *red, int cover) {
     constexpr int NT = kSpreadFxThreads, PPT = kSpreadFxSlots, CAP = NT * PPT;
     constexpr int Q0 = (NTAPS - 1) / 2, HALO = NTAPS;
     const int TT = 1 << A.tlog;
     const int NC = TT + 2 * HALO;  // cells -HALO .. TT+HALO-1
-    unsigned long long *acc = reinterpret_cast<unsigned long long *>(cuna_smem);
-    T *facc = reinterpret_cast<T *>(acc + NC);  // TT, for tiles of several rounds
-    __shared__ T red[NT / 64];
+    using Acc = typename FX::U;
+    using SAcc = std::make_signed_t<Acc>;
+    Acc *acc = reinterpret_cast<Acc *>(cuna_smem);
+    // TT, for tiles of several rounds (after room for 64-bit cells)
+    T *facc = reinterpret_cast<T *>(reinterpret_cast<unsigned long long *>(cuna_smem) + NC);
     // finite values whose products with cnt and wscale stay finite
     constexpr T kBig = sizeof(T) == 4 ? T(3.0e38f) : T(1.0e300);
     // 2^sexp also keeps wscale * 2^sexp finite
@@ -673,7 +737,7 @@ __global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A)
     const T wscale = A.cpar ? A.cpar[3 * c + 1] : A.wscale;
 
     const int tlen = min(TT, A.G - R.tstart);  // cells of the tile inside the grid
-    for (int g = tid; g < NC; g += NT) acc[g] = 0ull;
+    for (int g = tid; g < NC; g += NT) acc[g] = Acc(0);
     if (multi)
         for (int g = tid; g < TT; g += NT) facc[g] = T(0);
     // the cells a thread reads back were cleared by other threads: a tile
@@ -736,14 +800,14 @@ __global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A)
 #pragma unroll
         for (int w = 1; w < NT / 64; ++w) vmax = fmax(vmax, red[w]);
         if (!(vmax <= kBig)) finite = false;
-        // 2^sexp: cnt * vmax * wscale * 2^sexp <= 2^62, and wscale * 2^sexp <= 2^kMaxExp
-        const T bound = (T)cnt * vmax * wscale;
+        // 2^sexp: cnt * vmax * wscale * 2^sexp <= 2^headroom, and wscale * 2^sexp <= 2^kMaxExp
+        const T bound = (T)min(cnt, cover) * vmax * wscale;
         int eb = 0;
         if (bound > T(0) && finite) (void)frexp(bound, &eb);
         else eb = -64;
         int ew = 0;
         (void)frexp(wscale, &ew);
-        sexp = min(62 - eb, kMaxExp - ew);
+        sexp = min(FX::headroom - eb, kMaxExp - ew);
         const T mul = ldexp(wscale, sexp);
         if (finite) {
 #pragma unroll
@@ -757,12 +821,12 @@ __global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A)
                         for (int q = 0; q < NTAPS; ++q) {
                             const int cell = c0 + q;
                             if (cell >= -HALO && cell < TT + HALO)
-                                atomicAdd(&acc[HALO + cell], to_fixed64(w[q]));
+                                atomicAdd(&acc[HALO + cell], FX::fixed(w[q]));
                         }
                     } else {
-                        unsigned long long *dst = acc + HALO + c0;
+                        Acc *dst = acc + HALO + c0;
 #pragma unroll
-                        for (int q = 0; q < NTAPS; ++q) atomicAdd(dst + q, to_fixed64(w[q]));
+                        for (int q = 0; q < NTAPS; ++q) atomicAdd(dst + q, FX::fixed(w[q]));
                     }
                 }
             }
@@ -774,18 +838,35 @@ __global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A)
             // the reads of other waves), and the halo cells no thread reads
             const T inv = ldexp(T(1), -sexp);
             for (int g = tid; g < TT; g += NT) {
-                facc[g] += (T)(long long)acc[HALO + g] * inv;
-                acc[HALO + g] = 0ull;
+                facc[g] += (T)(SAcc)acc[HALO + g] * inv;
+                acc[HALO + g] = Acc(0);
             }
-            for (int h = tid; h < 2 * HALO; h += NT) acc[h < HALO ? h : TT + h] = 0ull;
+            for (int h = tid; h < 2 * HALO; h += NT) acc[h < HALO ? h : TT + h] = Acc(0);
             __syncthreads();
         }
     }
     T *gout = A.grid + (long)slab * A.grid_stride + R.tstart;
     const T inv = ldexp(T(1), -sexp);
     for (int g = tid; g < tlen; g += NT) {
-        const T v = multi ? facc[g] : (T)(long long)acc[HALO + g] * inv;
+        const T v = multi ? facc[g] : (T)(SAcc)acc[HALO + g] * inv;
         gout[g] = finite ? v : T(NAN);
+    }
+}
+
+// Scatter kernel: W32 (FP32 plans whose tiles all have cover <= kCover32)
+// uses 32-bit cells, else 64-bit.
+template <typename T, int NTAPS, int VM, bool W32>
+__global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A) {
+    __shared__ T red[kSpreadFxThreads / 64];
+    // no cell of the tile receives more than `cover` weights in a round
+    const int c = blockIdx.y / A.nrep;
+    const int cover = A.tile_cover ? A.tile_cover[(long)c * A.ntiles + blockIdx.x]
+                                   : kSpreadFxThreads * kSpreadFxSlots;
+    if constexpr (W32) {
+        static_assert(sizeof(T) == 4, "32-bit cells are for the FP32 library");
+        spread_fx_body<T, NTAPS, VM, Fixed32>(A, red, cover);
+    } else {
+        spread_fx_body<T, NTAPS, VM, Fixed64<T>>(A, red, cover);
     }
 }
 
