@@ -145,7 +145,7 @@ def cunfft_adjoint_raw_async(x, f_data, f_hat, n: int, ng: int, fprops, stream=N
 
 def cunfft_adjoint_raw_async_bootstrap(x, f_data, f_hat, n: int, ng: int, nbs: int, fprops,
                                        stream=None, seed: int = 0):
-    """src/cuna_kernel.cu:390-443 (the defined 9-argument form)."""
+    """src/cuna_kernel.cu:236-289 (the defined 9-argument form)."""
     L = load(_prec_of(x))
     L.cunfft_adjoint_raw_async_bootstrap(
         _ptr(x), _ptr(f_data), _ptr(f_hat), n, ng, nbs,

@@ -296,15 +296,21 @@ void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s, hipEv
 template <typename T>
 void Plan<T>::adjoint(const T *y, cplx<T> *out, hipStream_t s) {
     require(out != nullptr, "output pointer is NULL");
+    grid.ensure(sizeof(T) * (size_t)G * ncurves());
+    adjoint_into(y, out, grid.as<T>(), s);
+}
+
+template <typename T>
+void Plan<T>::adjoint_into(const T *y, cplx<T> *out, T *grid_ptr, hipStream_t s) const {
+    require(out != nullptr, "output pointer is NULL");
     const int nc = ncurves();
-    grid.ensure(sizeof(T) * (size_t)G * nc);
     SpreadArgs<T> A = spread_args(pts);
     A.y = y;
-    A.grid = grid.as<T>();
+    A.grid = grid_ptr;
     A.grid_stride = G;
     // prof[0] and prof[1] take the spread kernel's own start and stop times
     launch_spread<T>(A, ntaps, VAL_PLAIN, nc, s, prof[0], prof[1]);
-    run_fft(grid.as<T>(), out, nc, s, prof[2]);
+    run_fft(grid_ptr, out, nc, s, prof[2]);
     if (prof[3]) CUNA_HIP(hipEventRecord(prof[3], s));
 }
 
@@ -629,6 +635,7 @@ std::unique_ptr<Plan<T>> make_plan_accurate(const T *x, const double *t, double 
         }
         P->has_pts2 = true;
     }
+    TwiddleCache<T>::get().warm(P->lgN);
     return P;
 }
 
@@ -691,6 +698,7 @@ std::unique_ptr<Plan<T>> make_plan_compat(const T *x, int n, int ng, T *E1, T *E
     int tl = spec.tile_log2 > 0 ? spec.tile_log2 : default_tlog<T>();
     std::vector<int> off = {0, n};
     P->pts.build(A, off, ng, tl, P->ntaps, false, true, s);
+    TwiddleCache<T>::get().warm(P->lgN);
     return P;
 }
 
@@ -1154,7 +1162,7 @@ CUNA_API void cunfft_adjoint_from_plan(plan *p) {
             raw_impl(p->g_x_data, nullptr, p->g_f_hat_win, p->Ndata, p->Ngrid, p->fprops_device,
                      nullptr);
         // the reference meant to copy back unless DONT_TRANSFER_TO_CPU
-        // (src/cuna_kernel.cu:296; its test `!(flags | ...)` is always false)
+        // (src/cuna_kernel.cu:142; its test `!(flags | ...)` is always false)
         if (!(p->flags & DONT_TRANSFER_TO_CPU)) {
             CUNA_HIP(hipMemcpy(p->f_hat, p->g_f_hat, sizeof(Complex) * p->Ngrid,
                                hipMemcpyDeviceToHost));

@@ -2,7 +2,7 @@
 //
 // Replaces the cuFFT stage of the reference (cufftExec{C2C,Z2Z} with
 // CUFFT_INVERSE, unnormalised, in place: src/cuna_kernel.cu:43-49,80-99,
-// 321-333,360-374,411-430).  The grid of the adjoint NFFT is real, so a real
+// 167-178,206-220,257-276).  The grid of the adjoint NFFT is real, so a real
 // grid of G cells is transformed as a complex sequence of N = G/2 values
 // z[n] = grid[2n] + i grid[2n+1] (the same memory), and the spectrum is
 // recovered pairwise from Z[r], Z[N-r]; the window deconvolution is applied
@@ -16,6 +16,7 @@
 
 #include <type_traits>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -328,7 +329,25 @@ class TwiddleCache {
         return t;
     }
 
+    // Every table an FFT of N = 2^lgN complex values (and its real-input
+    // epilogue, G = 2N) may use, on the current device.
+    void warm(int lgN) {
+        for (int l = 1; l <= std::min(lgN, 14); ++l) fine(l);
+        two_level(lgN);
+        two_level(lgN + 1);
+    }
+    // While a NoAlloc guard lives on this thread, a table not yet cached is
+    // an error instead of a new allocation.
+    struct NoAlloc {
+        NoAlloc() { flag() = true; }
+        ~NoAlloc() { flag() = false; }
+    };
+
   private:
+    static bool &flag() {
+        static thread_local bool f = false;
+        return f;
+    }
     std::mutex mu_;
     std::map<std::tuple<int, int, int>, void *> tabs_;
     const cplx<T> *table(int lg, int kind, long count) {
@@ -338,6 +357,8 @@ class TwiddleCache {
         auto key = std::make_tuple(dev, lg, kind);
         auto it = tabs_.find(key);
         if (it != tabs_.end()) return static_cast<const cplx<T> *>(it->second);
+        require(!flag(), "twiddle table not prepared for this transform size",
+                CUNA_ERROR_INVALID_VALUE);
         std::vector<cplx<T>> h(count);
         const long double two_pi = 6.283185307179586476925286766559005768L;
         long N = 1L << lg;

@@ -9,21 +9,33 @@
 
 namespace cuna {
 
-// raise the dynamic LDS limit of a kernel once (not on every launch, so that
-// launches can be captured into a graph)
+// raise the dynamic LDS limit of a kernel once per device (not on every
+// launch, so that launches can be captured into a graph).  The record is a
+// fixed table: the lookup never allocates host memory (a full table only
+// means the attribute is set again on each launch).
 template <typename K>
 static void set_lds(K kernel, size_t bytes) {
     if (bytes <= 64 * 1024) return;
+    struct Entry {
+        const void *key;
+        int dev;
+        size_t bytes;
+    };
+    constexpr int kEntries = 512;
     static std::mutex mu;
-    static std::map<std::pair<const void *, int>, size_t> done;
+    static Entry done[kEntries];
+    static int used = 0;
     const void *key = reinterpret_cast<const void *>(kernel);
     int dev = 0;
     CUNA_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(mu);
-    auto &have = done[{key, dev}];
-    if (have >= bytes) return;
+    Entry *e = nullptr;
+    for (int i = 0; i < used && !e; ++i)
+        if (done[i].key == key && done[i].dev == dev) e = &done[i];
+    if (e && e->bytes >= bytes) return;
     CUNA_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    have = bytes;
+    if (!e && used < kEntries) e = &done[used++];
+    if (e) *e = Entry{key, dev, bytes};
 }
 
 static constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -40,6 +52,8 @@ constexpr int seq_per_wg(int lg, int cap) {
 }
 // two-pass split N = L1 * L2, L2 >= L1
 constexpr int split_lg1(int lgN) { return lgN / 2; }
+// largest log2 N of the single-workgroup inverse (Hermitian) transform
+constexpr int kC2rSmallLg = 13;
 
 template <typename T, int S, int LG, class Loader, class Sink, int MODE>
 static void run_small(const FftShape &sh, const Loader &ld, const Sink &sk, cplx<T> *plain_out,
@@ -114,19 +128,20 @@ static int env_int(const char *name) {
     return v ? atoi(v) : 0;
 }
 
-// compute units of the current device (cached per device)
+// compute units of the current device (cached per device, fixed table)
 static int num_cus() {
+    constexpr int kDevices = 64;
     static std::mutex mu;
-    static std::map<int, int> cache;
+    static int cache[kDevices] = {};
     int dev = 0;
     CUNA_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find(dev);
-    if (it != cache.end()) return it->second;
+    if (dev >= 0 && dev < kDevices && cache[dev] > 0) return cache[dev];
     int n = 0;
     CUNA_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
-    cache[dev] = std::max(n, 1);
-    return cache[dev];
+    n = std::max(n, 1);
+    if (dev >= 0 && dev < kDevices) cache[dev] = n;
+    return n;
 }
 
 template <typename T>
@@ -237,10 +252,11 @@ void fft_c2r_inverse(const FftShape &sh, const LoadHermitian<T> &ld, cplx<T> *wo
             CUNA_ERROR_UNSUPPORTED);
     cplx<T> *z = reinterpret_cast<cplx<T> *>(grid);
     SinkAccurate<T> dummy{};
-    // the Hermitian loader is register-heavy: the largest single-workgroup
-    // size (1024 threads) would spill, so it takes the two-pass path
-    if (sh.lgN < max_small_lg<T>()) {
-        dispatch_lg<1, max_small_lg<T>() - 1>(sh.lgN, [&](auto c) {
+    // the Hermitian loader is register-heavy: a 1024-thread workgroup (N =
+    // 2^14) would spill, so N >= 2^14 takes the two-pass path in both
+    // libraries, whose first split then has L1 >= 2^7
+    if (sh.lgN <= kC2rSmallLg) {
+        dispatch_lg<1, kC2rSmallLg>(sh.lgN, [&](auto c) {
             run_small<T, -1, decltype(c)::value, LoadHermitian<T>, SinkAccurate<T>, 1>(
                 sh, ld, dummy, z, s);
         });

@@ -104,7 +104,10 @@ struct Plan {
 
     // y -> out (ncurves x M complex)
     void adjoint(const T *y, cplx<T> *out, hipStream_t s);
-    // compat bootstrap of src/cuna_kernel.cu:390-443 (single curve)
+    // the same with the real grid (ncurves x G) in caller-provided memory;
+    // allocates nothing
+    void adjoint_into(const T *y, cplx<T> *out, T *grid_ptr, hipStream_t s) const;
+    // compat bootstrap of src/cuna_kernel.cu:236-289 (single curve)
     void bootstrap_compat(const T *y, cplx<T> *out, int nbs, uint32_t seed, hipStream_t s);
     // accurate bootstrap (resampled points)
     void bootstrap(const T *y, int nbs, uint32_t seed, cplx<T> *out_y, cplx<T> *out_w,

@@ -119,26 +119,22 @@ __device__ __forceinline__ void setup_point(const SetupArgs<T> &A, long i, int &
         }
         if (A.eps_orig) A.eps_orig[i] = e;
     } else {
-        // accurate: t = G*x mod G, anchor floor(t), e = t - floor(t)
-        double xs;
-        if (A.mode == SETUP_ACC_T) {
-            double s = A.df * A.t[i];
-            xs = s - floor(s);  // [0, 1): FP64 phase reduction
-        } else {
-            double xv = (double)A.x[i];
-            xs = xv - floor(xv);
-        }
-        double tg = xs * (double)A.G;
-        double c = floor(tg);
-        int ci = (int)c;
+        // accurate: t = G*x, anchor floor(t) mod G, e = t - floor(t).  With G
+        // a power of two, G*x and t - floor(t) are exact in FP64 and the
+        // anchor is reduced in integers: no bit of x is lost (reducing x to
+        // [0, 1) first would round x + 1 for negative x).  Times: x = df*t
+        // in FP64 (one rounding).
+        const double xv = A.mode == SETUP_ACC_T ? A.df * A.t[i] : (double)A.x[i];
+        const double tg = xv * (double)A.G;
+        const double c = floor(tg);
         T f = (T)(tg - c);
+        // (non-finite x: anchor 0 and a NaN e, which makes the output NaN)
+        long long ci = fabs(c) < 0x1p62 ? (long long)c : 0;
         if (f >= T(1)) {  // rounding of tg - c to T
             f = T(0);
             ci += 1;
         }
-        if (ci >= A.G) ci -= A.G;
-        if (!(ci >= 0 && ci < A.G)) ci = 0;  // non-finite x: keep the index in range
-        a = ci;
+        a = (int)(ci & (long long)(A.G - 1));  // G is a power of two
         e = f;
     }
 }

@@ -9,7 +9,7 @@
  * All functions compute the reference's transform
  *     f_hat[r] = normfac * exp((r/ng)^2 b/4) * sum_g grid[g] exp(+2 pi i g r/ng),
  *     grid[g]  = sum_j y_j exp(-(eps_j - m)^2/b)/sqrt(pi b),  g = j_j + m,
- * r = 0..ng-1 (see src/gridding.cu:36-65, src/cuna_kernel.cu:303-337,
+ * r = 0..ng-1 (see src/gridding.cu:36-65, src/cuna_kernel.cu:148-183,
  * src/gaussian_filter.cu:278-289).  ng must be a power of two >= 4.
  * Errors are recorded (cuna_get_last_error) instead of calling exit().
  */
@@ -26,26 +26,26 @@ extern "C" {
  * reference (whose copy-back never runs), fft is written. */
 void cunfft_adjoint(const dTyp *x, const dTyp *f, cTyp *fft, int n, int ng, unsigned int flags);
 
-/* inc/cuna.h:41, src/cuna_kernel.cu:291-299.  Runs the plan built by init_plan;
+/* inc/cuna.h:41, src/cuna_kernel.cu:137-145.  Runs the plan built by init_plan;
  * the result is left in p->g_f_hat (and p->g_f_hat_win with
  * CALCULATE_WINDOW_FUNCTION) and copied to p->f_hat unless DONT_TRANSFER_TO_CPU. */
 void cunfft_adjoint_from_plan(plan *p);
 
-/* inc/cuna.h:48-49, src/cuna_kernel.cu:303-337.  Device pointers; gpu_fprops
+/* inc/cuna.h:48-49, src/cuna_kernel.cu:148-183.  Device pointers; gpu_fprops
  * from generate_filter_properties for the same x, n, ng.  Runs on the null
  * stream.  f_hat[ng] is overwritten (the reference accumulated into a
  * caller-zeroed f_hat; the results agree for a zeroed f_hat). */
 void cunfft_adjoint_raw(const dTyp *x, const dTyp *f_data, Complex *f_hat, const int n,
                         const int ng, const filter_properties *gpu_fprops);
 
-/* inc/cuna.h:51-53, src/cuna_kernel.cu:341-386.  As above, ordered on stream;
+/* inc/cuna.h:51-53, src/cuna_kernel.cu:187-232.  As above, ordered on stream;
  * never blocks the host. */
 void cunfft_adjoint_raw_async(const dTyp *x, const dTyp *f_data, Complex *f_hat, const int n,
                               const int ng, const filter_properties *gpu_fprops,
                               hipStream_t stream);
 
 /* inc/cuna.h:55-56 declares this without seed; the reference DEFINES it with
- * seed (src/cuna_kernel.cu:390-443) and that definition is what is exported
+ * seed (src/cuna_kernel.cu:236-289) and that definition is what is exported
  * here.  f_hat holds nbs slabs of ng values: slab s is the transform of
  * bootstrap replicate s (values y[d], d = xorshift(wang(seed + s*n + p)) % n). */
 void cunfft_adjoint_raw_async_bootstrap(const dTyp *x, const dTyp *f_data, Complex *f_hat,

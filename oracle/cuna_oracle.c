@@ -280,7 +280,7 @@ static void compat_normalize(double *F, int ng, double b, double normfac) {
     }
 }
 
-/* src/cuna_kernel.cu:303-337: grid -> in-place inverse FFT -> normalize */
+/* src/cuna_kernel.cu:148-183: grid -> in-place inverse FFT -> normalize */
 int orc_compat_adjoint(const double *x, const double *y, int n, int ng, int dbl, double *out) {
     double *grid = (double *)malloc(sizeof(double) * (size_t)ng);
     if (orc_compat_grid(x, y, n, ng, dbl, grid)) {
@@ -300,7 +300,7 @@ int orc_compat_adjoint(const double *x, const double *y, int n, int ng, int dbl,
     return 0;
 }
 
-/* src/cuna_kernel.cu:391-443: bootstrap spread, batched FFT (dist ng),
+/* src/cuna_kernel.cu:237-289: bootstrap spread, batched FFT (dist ng),
  * normalize_bootstrap (src/gaussian_filter.cu:291-302, k = i % ng) */
 int orc_compat_adjoint_bootstrap(const double *x, const double *y, int n, int ng, int nbs,
                                  uint32_t seed, int dbl, double *out) {
@@ -352,12 +352,13 @@ static void gl_spread(const double *x, const double *y, int n, long G, int m, do
         double *pg = priv + (size_t)tid * (size_t)G;
 #pragma omp for schedule(static)
         for (int j = 0; j < n; j++) {
+            /* t = G x is exact (G a power of two), and so is t - floor(t);
+             * the anchor is reduced mod G in integers.  (Shifting a negative
+             * t by G first would round away its last bit.) */
             double t = (double)G * x[j];
-            t -= floor(t / (double)G) * (double)G; /* t in [0, G) */
-            if (t >= (double)G) t -= (double)G;
             double c = floor(t);
             double d = t - c;
-            long ci = (long)c;
+            long ci = (long)c % G;
             double v = y ? y[j] : 1.0;
             for (int o = -m + 1; o <= m; o++) {
                 double s = (double)o - d;
@@ -424,11 +425,9 @@ int orc_gl_grad(const double *x, const double *y, int n, int M, int sigma, int m
     orc_fft(h, (int)G, -1);
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++) {
-        double t = (double)G * x[j];
-        t -= floor(t / (double)G) * (double)G;
-        if (t >= (double)G) t -= (double)G;
+        double t = (double)G * x[j]; /* exact, as in gl_spread */
         double c = floor(t), d = t - c;
-        long ci = (long)c;
+        long ci = (long)c % G;
         double sw = 0.0, sd = 0.0;
         for (int o = -m + 1; o <= m; o++) {
             double s = (double)o - d;

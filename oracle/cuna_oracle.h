@@ -34,7 +34,7 @@ int orc_compat_grid(const double *x, const double *y, int n, int ng, int dbl, do
 /* nbs slabs of ng (src/gridding.cu:86-117) */
 int orc_compat_grid_bootstrap(const double *x, const double *y, int n, int ng, int nbs,
                               uint32_t seed, int dbl, double *grid);
-/* full compat transform: ng complex outputs (src/cuna_kernel.cu:303-337) */
+/* full compat transform: ng complex outputs (src/cuna_kernel.cu:148-183) */
 int orc_compat_adjoint(const double *x, const double *y, int n, int ng, int dbl, double *out);
 int orc_compat_adjoint_bootstrap(const double *x, const double *y, int n, int ng, int nbs,
                                  uint32_t seed, int dbl, double *out);

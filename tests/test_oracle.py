@@ -126,6 +126,24 @@ def test_gl_vs_ndft(m, tol):
     assert err > 1e-16
 
 
+def test_gl_keeps_every_bit_of_negative_fp64_x():
+    """At M = 2^20 the phase of the top modes needs every bit of x: shifting a
+    negative x by 1 (or G x by G) before the grid reduction drops its last
+    bit, an error of ~1e-10 at |k| ~ 2^19.  m = 16 keeps the method's own
+    error below 1e-13, so 1e-12 separates the two."""
+    rng = np.random.default_rng(11)
+    n, M = 4000, 1 << 20
+    x = rng.uniform(-0.5, 0.5, n) * (2.0 / 3.0)  # every mantissa bit in use
+    y = rng.standard_normal(n)
+    neg = x[x < 0]
+    assert np.mean((neg + 1.0) - 1.0 != neg) > 0.3  # x + 1 would round these
+    got = oracle.gl_adjoint(x, y, M, 2, 16)
+    k0 = np.array([-M // 2, M // 2 - 64, 1000], dtype=np.int64)
+    ref = oracle.ndft_bands(x, y, k0, 64)
+    sel = np.stack([got[k + M // 2:k + M // 2 + 64] for k in k0])
+    assert rel_l2(sel, ref) < 1e-12
+
+
 def test_gl_sigma4_and_unit_weights():
     x, y = uniform_points(1000, 6, np.float64)
     M = 256
