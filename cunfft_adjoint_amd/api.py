@@ -460,7 +460,7 @@ def capture(fn, stream, prec: str = "f") -> Graph:
 
 
 def fft(data, sign: int = +1, stream=None):
-    """In-place unnormalised DFT along the last axis (power-of-two length)."""
+    """In-place unnormalised DFT along the last axis (any length)."""
     import torch
 
     prec = "f" if data.dtype == torch.complex64 else "d"
@@ -473,6 +473,7 @@ def fft(data, sign: int = +1, stream=None):
 __all__ = [
     "CunaError", "cunfft_adjoint", "FilterProperties", "generate_filter_properties",
     "generate_pinned_filter_properties", "cunfft_adjoint_raw", "cunfft_adjoint_raw_async",
-    "cunfft_adjoint_raw_async_bootstrap", "CunaPlan", "NFFTPlan", "fft", "Graph", "capture",
+    "cunfft_adjoint_raw_async_bootstrap", "CunaPlan",
+    "NFFTPlan", "fft", "Graph", "capture",
     "autotune",
 ]

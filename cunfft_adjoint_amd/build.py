@@ -1,6 +1,6 @@
 """Build libcunaf.so (float) and libcunad.so (double) for gfx950 with hipcc.
 
-    python -m cunfft_adjoint_amd.build [--jobs N] [--only f|d]
+    python -m cunfft_adjoint_amd.build [--jobs N] [--only f|d] [--incremental]
 
 Both libraries come from the same HIP sources; the double build adds
 -DDOUBLE_PRECISION, as the reference's Makefile does for libcunad.so.
@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIBDIR = os.path.join(HERE, "lib")
 BUILDDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["cuna_kernels.hip", "cuna_api.hip", "cuna_tune.hip"]
+SOURCES = ["cuna_kernels.hip", "cuna_api.hip", "cuna_tune.hip", "cuna_chirp.hip"]
 ARCH = os.environ.get("CUNA_OFFLOAD_ARCH", "gfx950")
 
 # packed-fp32 instructions (v_pk_*_f32) are disabled for all device code: some
@@ -50,7 +50,18 @@ def _run(cmd):
     return r.stdout
 
 
-def build(jobs: int = 4, only: str | None = None, verbose: bool = False) -> list[str]:
+def _newest_input(src: str) -> float:
+    """Latest mtime of a source and every header it may include."""
+    heads = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    heads += [os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE) if f.endswith(".h")]
+    return max(os.path.getmtime(p) for p in [src, __file__, *heads])
+
+
+def build(jobs: int = 4, only: str | None = None, verbose: bool = False,
+          incremental: bool = False) -> list[str]:
+    """Compile both libraries.  incremental=True skips objects newer than
+    their source and every header (a development shortcut; the default
+    always recompiles)."""
     cc = hipcc()
     os.makedirs(LIBDIR, exist_ok=True)
     tasks = []
@@ -61,7 +72,11 @@ def build(jobs: int = 4, only: str | None = None, verbose: bool = False) -> list
         os.makedirs(bdir, exist_ok=True)
         for src in SOURCES:
             obj = os.path.join(bdir, src.replace(".hip", ".o"))
-            cmd = [cc, *COMMON, *defs, "-c", os.path.join(CSRC, src), "-o", obj]
+            spath = os.path.join(CSRC, src)
+            if (incremental and os.path.exists(obj)
+                    and os.path.getmtime(obj) > _newest_input(spath)):
+                continue
+            cmd = [cc, *COMMON, *defs, "-c", spath, "-o", obj]
             tasks.append((tag, obj, cmd))
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         futs = {ex.submit(_run, cmd): (tag, obj, cmd) for tag, obj, cmd in tasks}
@@ -87,8 +102,10 @@ def main(argv=None):
     ap.add_argument("--jobs", type=int, default=4)
     ap.add_argument("--only", choices=["f", "d"])
     ap.add_argument("-v", action="store_true")
+    ap.add_argument("--incremental", action="store_true",
+                    help="skip objects newer than their inputs")
     a = ap.parse_args(argv)
-    for p in build(a.jobs, a.only, a.v):
+    for p in build(a.jobs, a.only, a.v, a.incremental):
         print(p)
 
 

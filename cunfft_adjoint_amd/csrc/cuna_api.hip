@@ -263,6 +263,18 @@ SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
 }
 
 template <typename T>
+void Plan<T>::prepare_fft(hipStream_t s) {
+    if (pow2_grid(G)) {
+        TwiddleCache<T>::get().warm(lgN);
+        return;
+    }
+    require(spec.mode == CUNA_MODE_COMPAT, "accurate-mode grids are powers of two");
+    blu = std::make_unique<Bluestein<T>>();
+    blu->init(G, s);
+    blu->ensure(std::max(ncurves(), 1));
+}
+
+template <typename T>
 void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s, hipEvent_t mid) const {
     FftShape sh;
     sh.lgN = lgN;
@@ -289,7 +301,10 @@ void Plan<T>::run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s, hipEv
         sk.normfac = normfac_c;
         sk.row_off = row_off_c;
         sk.qb_row = qb_row_c;
-        fft_real_forward<T>(sh, grid_ptr, sk, s, mid);
+        if (blu)
+            blu->run_real(grid_ptr, G, batch, sk, s);
+        else
+            fft_real_forward<T>(sh, grid_ptr, sk, s, mid);
     }
 }
 
@@ -320,6 +335,7 @@ void Plan<T>::bootstrap_compat(const T *y, cplx<T> *out, int nbs, uint32_t seed,
     require(nbs > 0, "nbs must be positive");
     require(out != nullptr, "output pointer is NULL");
     grid.ensure(sizeof(T) * (size_t)G * nbs);
+    if (blu) blu->ensure(nbs);
     SpreadArgs<T> A = spread_args(pts);
     A.y = y;
     A.grid = grid.as<T>();
@@ -635,7 +651,7 @@ std::unique_ptr<Plan<T>> make_plan_accurate(const T *x, const double *t, double 
         }
         P->has_pts2 = true;
     }
-    TwiddleCache<T>::get().warm(P->lgN);
+    P->prepare_fft(s);
     return P;
 }
 
@@ -660,12 +676,20 @@ void compat_constants(int n, int ng, T &b, T &normfac, int &R_f) {
     }
 }
 
+// any ng >= 1 whose transform fits: powers of two >= 4 through the real
+// Stockham FFT, other lengths through Bluestein at L >= 2 ng - 1
+template <typename T>
+static bool compat_ng_ok(long ng) {
+    if (ng < 1 || ng > (1L << 26)) return false;
+    if (pow2_grid(ng)) return ilog2(ng) - 1 <= max_fft_lg<T>();
+    return ilog2(2 * ng - 1) <= max_fft_lg<T>();
+}
+
 template <typename T>
 std::unique_ptr<Plan<T>> make_plan_compat(const T *x, int n, int ng, T *E1, T *E2, T *E3,
                                           const ModeSpec &spec, hipStream_t s) {
     require(n > 0, "n must be positive");
-    require(ng >= 4 && is_pow2(ng), "ng must be a power of two >= 4", CUNA_ERROR_UNSUPPORTED);
-    require(ilog2(ng) - 1 <= max_fft_lg<T>(), "ng too large", CUNA_ERROR_UNSUPPORTED);
+    require(compat_ng_ok<T>(ng), "ng too large", CUNA_ERROR_UNSUPPORTED);
     // b = 2R/(2R-1) R_f/pi with R = ng/n is negative (NaN weights) for n >= 2 ng
     require((long)n < 2L * ng, "n must be smaller than 2*ng (the filter width b is not positive)");
     require(x != nullptr, "x is NULL");
@@ -674,7 +698,7 @@ std::unique_ptr<Plan<T>> make_plan_compat(const T *x, int n, int ng, T *E1, T *E
     P->spec.mode = CUNA_MODE_COMPAT;
     P->M = ng;
     P->G = ng;
-    P->lgN = ilog2(ng) - 1;
+    P->lgN = pow2_grid(ng) ? ilog2(ng) - 1 : 0;
     compat_constants<T>(n, ng, P->b_c, P->normfac_c, P->R_f);
     P->ntaps = 2 * P->R_f - 1;
     P->olo = -(P->R_f - 1);
@@ -698,7 +722,7 @@ std::unique_ptr<Plan<T>> make_plan_compat(const T *x, int n, int ng, T *E1, T *E
     int tl = spec.tile_log2 > 0 ? spec.tile_log2 : default_tlog<T>();
     std::vector<int> off = {0, n};
     P->pts.build(A, off, ng, tl, P->ntaps, false, true, s);
-    TwiddleCache<T>::get().warm(P->lgN);
+    P->prepare_fft(s);
     return P;
 }
 
@@ -755,10 +779,11 @@ static std::shared_ptr<Plan<R>> compat_plan_for(const R *x, int n, int ng,
     return std::shared_ptr<Plan<R>>(std::move(fresh));
 }
 
-// the reference accepts any ng for cuFFT; this library's FFT takes powers of two
+// the reference accepts any ng (cuFFT takes any length), and so does this
+// library up to the FFT's largest size
 static void check_compat_sizes(int n, int ng) {
     require(n > 0 && ng > 0, "n and ng must be positive");
-    require(ng >= 4 && is_pow2(ng), "ng must be a power of two >= 4", CUNA_ERROR_UNSUPPORTED);
+    require(compat_ng_ok<R>(ng), "ng too large", CUNA_ERROR_UNSUPPORTED);
     require((long)n < 2L * ng, "n must be smaller than 2*ng (the filter width b is not positive)");
 }
 
@@ -817,7 +842,6 @@ static std::unique_ptr<CompatBatch> make_compat_batch(const R *x, const int *n, 
     std::vector<long> off(nlc + 1, 0);
     for (int i = 0; i < nlc; ++i) {
         check_compat_sizes(n[i], ng[i]);
-        require(ilog2(ng[i]) - 1 <= max_fft_lg<R>(), "ng too large", CUNA_ERROR_UNSUPPORTED);
         off[i + 1] = off[i] + n[i];
         B->idist = std::max(B->idist, ng[i]);
     }
@@ -851,7 +875,7 @@ static std::unique_ptr<CompatBatch> make_compat_batch(const R *x, const int *n, 
         P->spec.mode = CUNA_MODE_COMPAT;
         P->M = g;
         P->G = g;
-        P->lgN = ilog2(g) - 1;
+        P->lgN = pow2_grid(g) ? ilog2(g) - 1 : 0;
         compat_constants<R>(n[cur[0]], g, P->b_c, P->normfac_c, P->R_f);
         P->ntaps = 2 * P->R_f - 1;
         P->olo = -(P->R_f - 1);
@@ -868,6 +892,7 @@ static std::unique_ptr<CompatBatch> make_compat_batch(const R *x, const int *n, 
         A.b = P->b_c;
         A.e1s = P->wscale;
         P->pts.build(A, goff, g, default_tlog<R>(), P->ntaps, false, false, s);
+        P->prepare_fft(s);
         hipLaunchKernelGGL(k_compose, dim3(ceil_div(std::max(gn, 1L), 256)), dim3(256), 0, s,
                            P->pts.perm.as<int>(), d_idx.as<int>(), gn);
         CUNA_LAUNCH_CHECK();
@@ -1455,8 +1480,20 @@ CUNA_API int cuna_graph_destroy(cuna_graph graph) {
 CUNA_API int cuna_fft(Complex *data, int n, int batch, int sign, hipStream_t stream) {
     return guarded([&] {
         require(data != nullptr, "data is NULL");
-        require(n >= 2 && is_pow2(n), "n must be a power of two >= 2", CUNA_ERROR_UNSUPPORTED);
+        require(n >= 1, "n must be positive");
         require(batch >= 1, "batch must be positive");
+        require(sign == 1 || sign == -1, "sign must be +1 or -1");
+        if (n < 2 || !is_pow2(n)) {
+            // any other length: Bluestein, set up for this call (as the
+            // reference creates its cuFFT plan on every call); the call
+            // waits for its stream before the workspace is released
+            Bluestein<R> b;
+            b.init(n, stream);
+            b.ensure(batch);
+            b.run_complex(reinterpret_cast<cplx<R> *>(data), batch, sign, stream);
+            CUNA_HIP(hipStreamSynchronize(stream));
+            return;
+        }
         FftShape sh;
         sh.lgN = ilog2(n);
         sh.batch = batch;

@@ -70,6 +70,38 @@ struct ModeSpec {
     int fft_w = 0, fft_h = 0;  // FFT workgroup shape (0: heuristic)
 };
 
+// Bluestein (chirp-z) transform of length ng, for grids whose length is not
+// a power of two >= 4 (the reference hands any ng to cuFFT,
+// src/cuna_kernel.cu:82,169,208,261):
+//   A[r] = sum_g z_g e^{+2 pi i g r/ng} = c_r sum_g (z_g c_g) conj(c_{r-g}),
+//   c_k = exp(+i pi k^2/ng),
+// a circular convolution of length L = 2^lgL >= 2 ng - 1 done with the
+// power-of-two FFT: conv = FFT_+(FFT_-(z c) * FFT_-(b)) / L, with
+// b_k = b_{L-k} = conj(c_k) for 0 <= k < ng and zero between.
+template <typename T>
+struct Bluestein {
+    // host sources of the (async) uploads, declared first so that they are
+    // destroyed after the device buffers (whose hipFree synchronises)
+    std::vector<cplx<T>> chirp_h, b_h;
+    int ng = 0, lgL = 0;
+    long L = 0;
+    DevMem chirp;  // c_k, k < ng
+    DevMem bhat;   // FFT_-(b), L values
+    DevMem work;   // batch_cap x L complex for the data and as much for the FFT passes
+    int batch_cap = 0;
+    void init(int ng_, hipStream_t s);
+    void ensure(int batch);
+    // real grids (batch rows of stride gstride, the first ng cells used) ->
+    // the compat deconvolution sink, one row per grid
+    void run_real(const T *grid, long gstride, int batch, const SinkCompat<T> &sk,
+                  hipStream_t s);
+    // in place on batch x ng complex values: data[r] = sum_g data[g] e^{sign 2 pi i g r/ng}
+    void run_complex(cplx<T> *data, int batch, int sign, hipStream_t s);
+};
+
+// true when a real grid of G cells takes the power-of-two FFT (else Bluestein)
+inline bool pow2_grid(long G) { return G >= 4 && is_pow2(G); }
+
 template <typename T>
 struct Plan {
     ModeSpec spec;
@@ -91,6 +123,7 @@ struct Plan {
     TiledPoints<T> pts, pts2;  // pts2: points 2x (window at 2f)
     bool has_pts2 = false;
     DevMem grid, work, counts, counter;
+    std::unique_ptr<Bluestein<T>> blu;  // compat grids of any other length
     // fused bootstrap: per-curve max |y| and densest tap window of x and 2x
     DevMem ymax, cover;
     DevMem ls_work;  // Lomb-Scargle: window spectra of the resident workgroups
@@ -127,6 +160,9 @@ struct Plan {
     BootArgs<T> fused_args(const T *y, int nbs, uint32_t seed, hipStream_t s);
     void run_fft(T *grid_ptr, cplx<T> *out, int batch, hipStream_t s,
                  hipEvent_t mid = nullptr) const;
+    // twiddle tables (and the Bluestein set-up for other lengths) for this
+    // plan's transform; run once at plan creation
+    void prepare_fft(hipStream_t s);
 };
 
 template <typename T>

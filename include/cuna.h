@@ -10,7 +10,8 @@
  *     f_hat[r] = normfac * exp((r/ng)^2 b/4) * sum_g grid[g] exp(+2 pi i g r/ng),
  *     grid[g]  = sum_j y_j exp(-(eps_j - m)^2/b)/sqrt(pi b),  g = j_j + m,
  * r = 0..ng-1 (see src/gridding.cu:36-65, src/cuna_kernel.cu:148-183,
- * src/gaussian_filter.cu:278-289).  ng must be a power of two >= 4.
+ * src/gaussian_filter.cu:278-289).  Any ng >= 1 up to 2^23 (libcunaf) / 2^22
+ * (libcunad), and powers of two up to 2^25 / 2^24, as cuFFT takes any length.
  * Errors are recorded (cuna_get_last_error) instead of calling exit().
  */
 #ifndef CUNA_H

@@ -162,7 +162,7 @@ int cuna_graph_destroy(cuna_graph graph);
  * (src/cuna_kernel.cu:302-358 cunfft_adjoint_raw_batched_async,
  * src/gaussian_filter.cu:67-99 generate_pinned_filter_properties_batch).
  * Curve i has n[i] points of the concatenated x (and f_data; NULL = unit
- * weights) and ng[i] grid cells (power of two >= 4, n[i] < 2 ng[i]); its
+ * weights) and ng[i] grid cells (any ng[i] >= 1, n[i] < 2 ng[i]); its
  * ng[i] outputs (exactly those of cunfft_adjoint_raw_async on that curve)
  * go to the start of slab i of f_hat, slab stride idist = max ng, f_hat
  * holding nlc * idist values; the rest of each slab is left as it was.
@@ -190,7 +190,8 @@ void generate_pinned_filter_properties_batch(const dTyp *x, const int *n, const 
 void cuna_batch_cache_clear(void);
 
 /* The FFT stage alone: in-place unnormalised DFT of `batch` sequences of n
- * (power of two) values, data[r] = sum_g data[g] exp(sign*2 pi i g r/n).
+ * values (any n >= 1: powers of two by the Stockham passes, other lengths by
+ * Bluestein through them), data[r] = sum_g data[g] exp(sign*2 pi i g r/n).
  * sign = +1 is the reference's CUFFT_INVERSE. */
 int cuna_fft(Complex *data, int n, int batch, int sign, hipStream_t stream);
 

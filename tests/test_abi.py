@@ -134,10 +134,13 @@ def test_validation_errors_without_gpu(lib):
     assert L.cuna_nfft_plan_create(ctypes.byref(h), None, 0, 64, ctypes.addressof(p),
                                    None) == _lib.CUNA_ERROR_INVALID_VALUE
     p.tile_log2 = 0
-    # compat mode: ng not a power of two, n >= 2 ng (b would be negative), NULL x
+    # compat mode: ng beyond the FFT, n >= 2 ng (b would be negative), NULL x
+    # (any ng >= 1 of a supported size is accepted, as cuFFT takes any length)
     p.mode = 1
-    st = L.cuna_nfft_plan_create(ctypes.byref(h), None, 10, 100, ctypes.addressof(p), None)
+    st = L.cuna_nfft_plan_create(ctypes.byref(h), None, 10, 1 << 27, ctypes.addressof(p), None)
     assert st == _lib.CUNA_ERROR_UNSUPPORTED
+    st = L.cuna_nfft_plan_create(ctypes.byref(h), None, 10, 100, ctypes.addressof(p), None)
+    assert st == _lib.CUNA_ERROR_INVALID_VALUE and b"NULL" in L.cuna_get_last_error_string()
     st = L.cuna_nfft_plan_create(ctypes.byref(h), None, 64, 32, ctypes.addressof(p), None)
     assert st == _lib.CUNA_ERROR_INVALID_VALUE and b"2*ng" in L.cuna_get_last_error_string()
     st = L.cuna_nfft_plan_create(ctypes.byref(h), None, 10, 64, ctypes.addressof(p), None)
@@ -152,9 +155,11 @@ def test_validation_errors_without_gpu(lib):
     st = L.cuna_nfft_plan_create_batch(ctypes.byref(h), None, off, 2, 64, ctypes.addressof(p),
                                        None)
     assert st == _lib.CUNA_ERROR_INVALID_VALUE
-    # FFT stage: length not a power of two; NULL data
-    st = L.cuna_fft(ctypes.c_void_p(16), 12, 1, 1, None)
+    # FFT stage: a length whose Bluestein convolution exceeds the FFT; bad
+    # sign; NULL data
+    st = L.cuna_fft(ctypes.c_void_p(16), (1 << 24) + 1, 1, 1, None)
     assert st == _lib.CUNA_ERROR_UNSUPPORTED
+    assert L.cuna_fft(ctypes.c_void_p(16), 16, 1, 2, None) == _lib.CUNA_ERROR_INVALID_VALUE
     assert L.cuna_fft(None, 16, 1, 1, None) == _lib.CUNA_ERROR_INVALID_VALUE
     # reference entry points record errors instead of exiting
     L.cunfft_adjoint_raw(None, None, None, 10, 64, None)
@@ -166,8 +171,8 @@ def test_validation_errors_without_gpu(lib):
     assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE
     x = (R * 4)(0.0, 0.1, 0.2, 0.3)
     out = (R * 8)()
-    L.cunfft_adjoint(x, None, out, 4, 6, 0)  # ng = 6: not a power of two
-    assert L.cuna_get_last_error() != 0
+    L.cunfft_adjoint(x, None, out, 4, 2, 0)  # n = 2 ng: b would not be positive
+    assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE
     # batched compat API: sizes are checked before any device work
     fake = ctypes.c_void_p(256)
     n2, ng2 = (ctypes.c_int * 2)(5, 5), (ctypes.c_int * 2)(16, 16)
@@ -175,9 +180,9 @@ def test_validation_errors_without_gpu(lib):
     assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE  # f_hat NULL
     L.cunfft_adjoint_raw_batched_async(fake, None, fake, n2, ng2, 0, None, None)
     assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE  # no curves
-    L.cunfft_adjoint_raw_batched_async(fake, None, fake, n2, (ctypes.c_int * 2)(16, 24), 2,
-                                       None, None)
-    assert L.cuna_get_last_error() == _lib.CUNA_ERROR_UNSUPPORTED  # ng = 24
+    L.cunfft_adjoint_raw_batched_async(fake, None, fake, n2, (ctypes.c_int * 2)(16, 1 << 27),
+                                       2, None, None)
+    assert L.cuna_get_last_error() == _lib.CUNA_ERROR_UNSUPPORTED  # ng = 2^27
     L.cunfft_adjoint_raw_batched_async(fake, None, fake, (ctypes.c_int * 2)(5, 40), ng2, 2,
                                        None, None)
     assert L.cuna_get_last_error() == _lib.CUNA_ERROR_INVALID_VALUE  # n >= 2 ng
