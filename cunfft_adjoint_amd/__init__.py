@@ -16,6 +16,7 @@ from .api import (  # noqa: F401
     cunfft_adjoint_raw,
     cunfft_adjoint_raw_async,
     cunfft_adjoint_raw_async_bootstrap,
+    cunfft_adjoint_raw_async_noalloc,
     cunfft_adjoint_raw_batched_async,
     batch_cache_clear,
     fft,

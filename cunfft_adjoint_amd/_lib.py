@@ -84,6 +84,7 @@ def _protos(prec: str):
         "cunfft_adjoint_from_plan": (None, [P]),
         "cunfft_adjoint_raw": (None, [P, P, P, c_int, c_int, P]),
         "cunfft_adjoint_raw_async": (None, [P, P, P, c_int, c_int, P, P]),
+        "cunfft_adjoint_raw_async_noalloc": (None, [P, P, P, c_int, c_int, P, P]),
         "cunfft_adjoint_raw_async_bootstrap": (None, [P, P, P, c_int, c_int, c_int, P, P, c_uint]),
         # include/cuna_nfft.h: the reference's commented-out batched API
         "cunfft_adjoint_raw_batched_async": (None, [P, P, P, P, P, c_int, P, P]),

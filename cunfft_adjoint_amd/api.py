@@ -143,6 +143,17 @@ def cunfft_adjoint_raw_async(x, f_data, f_hat, n: int, ng: int, fprops, stream=N
     check(L)
 
 
+def cunfft_adjoint_raw_async_noalloc(x, f_data, f_hat, n: int, ng: int, fprops, stream=None):
+    """cunfft_adjoint_raw_async without any device allocation (include/cuna.h;
+    no counterpart in the reference): the grid lives in the library's fixed
+    per-GPU scratch array, ng <= 2^22 (float) / 2^21 (double)."""
+    L = load(_prec_of(x))
+    L.cunfft_adjoint_raw_async_noalloc(_ptr(x), _ptr(f_data), _ptr(f_hat), n, ng,
+                                       fprops.device if isinstance(fprops, FilterProperties)
+                                       else fprops, _stream(stream))
+    check(L)
+
+
 def cunfft_adjoint_raw_async_bootstrap(x, f_data, f_hat, n: int, ng: int, nbs: int, fprops,
                                        stream=None, seed: int = 0):
     """src/cuna_kernel.cu:236-289 (the defined 9-argument form)."""
@@ -473,7 +484,7 @@ def fft(data, sign: int = +1, stream=None):
 __all__ = [
     "CunaError", "cunfft_adjoint", "FilterProperties", "generate_filter_properties",
     "generate_pinned_filter_properties", "cunfft_adjoint_raw", "cunfft_adjoint_raw_async",
-    "cunfft_adjoint_raw_async_bootstrap", "CunaPlan",
+    "cunfft_adjoint_raw_async_bootstrap", "cunfft_adjoint_raw_async_noalloc", "CunaPlan",
     "NFFTPlan", "fft", "Graph", "capture",
     "autotune",
 ]

@@ -787,6 +787,36 @@ static void check_compat_sizes(int n, int ng) {
     require((long)n < 2L * ng, "n must be smaller than 2*ng (the filter width b is not positive)");
 }
 
+// ------------------------------------------------- allocation-free entry --
+// The entry cunfft_adjoint_raw_async_noalloc (below) keeps the real grid of
+// its transform in this file-scope device array instead of a plan workspace,
+// so the call allocates nothing.  Every GPU holds its own copy until the
+// process ends; all calls on one GPU share it (calls that overlap on a GPU
+// race).  The spread writes every cell of the grid, so nothing needs clearing.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device, resolved once per device
+// with hipGetSymbolAddress (in host code the array's name is only a
+// stand-in, not a device address).
+static void *scratch_ptr() {
+    constexpr int kDevices = 64;
+    static std::mutex mu;
+    static void *ptr[kDevices] = {};
+    int dev = 0;
+    CUNA_HIP(hipGetDevice(&dev));
+    require(dev >= 0 && dev < kDevices, "device index out of range for the scratch table",
+            CUNA_ERROR_UNSUPPORTED);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        CUNA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "hipGetSymbolAddress returned no address", CUNA_ERROR_HIP);
+        ptr[dev] = p;
+    }
+    return ptr[dev];
+}
+
 static void generate_impl(const R *x, int n, int ng, filter_properties *h, filter_properties *d,
                           hipStream_t s, bool sync) {
     require(h != nullptr && d != nullptr, "filter_properties pointers are NULL");
@@ -802,6 +832,10 @@ static void generate_impl(const R *x, int n, int ng, filter_properties *h, filte
     spec.mode = CUNA_MODE_COMPAT;
     auto p = make_plan_compat<R>(x, n, ng, h->E1, h->E2, h->E3, spec, s);
     p->x_key = x;
+    // set-up of the allocation-free transform on this device: resolving the
+    // scratch array's address loads its code object (the runtime allocates
+    // then), so that cunfft_adjoint_raw_async_noalloc itself allocates nothing
+    (void)scratch_ptr();
     if (sync) CUNA_HIP(hipStreamSynchronize(s));
     reg_put(d, std::move(p));
 }
@@ -1088,6 +1122,28 @@ CUNA_API void cunfft_adjoint_raw_async(const dTyp *x, const dTyp *f_data, Comple
                                        const int n, const int ng,
                                        const filter_properties *gpu_fprops, hipStream_t stream) {
     guarded([&] { raw_impl(x, f_data, f_hat, n, ng, gpu_fprops, stream); });
+}
+
+CUNA_API void cunfft_adjoint_raw_async_noalloc(const dTyp *x, const dTyp *f_data, Complex *f_hat,
+                                               const int n, const int ng,
+                                               const filter_properties *gpu_fprops,
+                                               hipStream_t stream) {
+    guarded([&] {
+        require(f_hat != nullptr, "f_hat is NULL");
+        require(n > 0 && ng > 0, "n and ng must be positive");
+        require((size_t)ng * sizeof(R) <= kScratchBytes,
+                "ng exceeds the scratch of the allocation-free transform (2^22 cells in "
+                "libcunaf, 2^21 in libcunad)",
+                CUNA_ERROR_UNSUPPORTED);
+        // the plan must exist already: building one allocates
+        auto p = gpu_fprops ? reg_get(gpu_fprops) : nullptr;
+        require(p && p->x_key == x && p->n() == n && p->G == ng,
+                "gpu_fprops must come from generate_filter_properties or "
+                "generate_pinned_filter_properties for this x, n and ng");
+        TwiddleCache<R>::NoAlloc guard;
+        p->adjoint_into(f_data, reinterpret_cast<cplx<R> *>(f_hat),
+                        static_cast<R *>(scratch_ptr()), stream);
+    });
 }
 
 CUNA_API void cunfft_adjoint_raw_async_bootstrap(const dTyp *x, const dTyp *f_data,

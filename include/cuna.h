@@ -45,6 +45,21 @@ void cunfft_adjoint_raw_async(const dTyp *x, const dTyp *f_data, Complex *f_hat,
                               const int ng, const filter_properties *gpu_fprops,
                               hipStream_t stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the transform of cunfft_adjoint_raw_async, with the
+ * same arguments and result, that calls no hipMalloc/hipFree.  Its grid
+ * lives in a 16 MiB device array of the library (one per GPU, held until the
+ * process ends), so ng is at most 2^22 (libcunaf) / 2^21 (libcunad); larger
+ * ng is rejected (CUNA_ERROR_UNSUPPORTED).  gpu_fprops must come from
+ * generate_filter_properties / generate_pinned_filter_properties for this
+ * x, n and ng (they build the plan; this call builds none).  All calls on
+ * one GPU share that array: calls that overlap on a GPU (other streams, other
+ * threads) race. */
+void cunfft_adjoint_raw_async_noalloc(const dTyp *x, const dTyp *f_data, Complex *f_hat,
+                                      const int n, const int ng,
+                                      const filter_properties *gpu_fprops, hipStream_t stream);
+
 /* inc/cuna.h:55-56 declares this without seed; the reference DEFINES it with
  * seed (src/cuna_kernel.cu:236-289) and that definition is what is exported
  * here.  f_hat holds nbs slabs of ng values: slab s is the transform of
