@@ -209,14 +209,6 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
         const size_t ncov = (size_t)std::max(ncurves, 1) * std::max(ntiles, 1);
         tile_cover.alloc(sizeof(int) * ncov);
         launch_tile_cover<T>(C, ncurves, tile_cover.as<int>(), s);
-        // the largest cover picks the spread's accumulator (read back once)
-        std::vector<int> hc(ncov, 0);
-        if (ncurves > 0) {
-            CUNA_HIP(hipMemcpyAsync(hc.data(), tile_cover.ptr, sizeof(int) * ncov,
-                                    hipMemcpyDeviceToHost, s));
-            CUNA_HIP(hipStreamSynchronize(s));
-        }
-        max_cover = *std::max_element(hc.begin(), hc.end());
     }
     // work-queue order of the curves for the fused batched kernels
     std::vector<int> ord(ncurves);
@@ -251,13 +243,13 @@ SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
     A.nrep = 1;
     A.periodic = P.periodic ? 1 : 0;
     A.wrap1 = (P.periodic && P.ntiles == 1) ? 1 : 0;
+    A.npts = (int)P.n;
     A.cap = sizeof(T) == 4 ? 2048 : 1024;
     A.gather = (spec.flags & CUNA_PLAN_GATHER) ? 1 : 0;
     A.bi = T(1) / beta;
     A.wscale = wscale;
     A.cpar = cpar_c;
     A.tile_cover = P.tile_cover.template as<int>();
-    A.fixed32 = (sizeof(T) == 4 && P.max_cover <= kCover32) ? 1 : 0;
     for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
     return A;
 }
@@ -586,11 +578,12 @@ void Plan<T>::forward(const cplx<T> *fhat, cplx<T> *f, hipStream_t s) {
     }
 }
 
-// 512-cell tiles: many small workgroups keep the whole chip busy and bound
-// the work of the densest tile
+// Spread tiles: 256 cells for the one-wave FP32 scatter, 512 for the FP64
+// workgroup scatter (the fastest measured at N = 1e6, G = 2^21); many small
+// tiles keep the chip busy and bound the work of the densest tile
 template <typename T>
 static int default_tlog() {
-    return 9;
+    return sizeof(T) == 4 ? 8 : 9;
 }
 
 template <typename T>
@@ -855,6 +848,11 @@ struct CompatBatch {
     std::vector<const void *> fps;  // device filter_properties per curve
     int idist = 0;
     struct Group {
+        // host sources of the group's async uploads, declared first so that
+        // they outlive the device buffers (whose hipFree synchronises)
+        std::vector<int> idx_h;
+        std::vector<R> cpar_h, qb_h;
+        std::vector<long> ro_h;
         std::unique_ptr<Plan<R>> plan;
         DevMem cpar, qb_row, row_off;
     };
@@ -887,8 +885,10 @@ static std::unique_ptr<CompatBatch> make_compat_batch(const R *x, const int *n, 
         const int g = kv.first;
         const std::vector<int> &cur = kv.second;
         const int nc = (int)cur.size();
+        auto G = std::make_unique<CompatBatch::Group>();
         // the group's points in curve order: their indices in x, and offsets
-        std::vector<int> gidx, goff(1, 0);
+        std::vector<int> &gidx = G->idx_h;
+        std::vector<int> goff(1, 0);
         for (int i : cur) {
             for (long j = off[i]; j < off[i + 1]; ++j) gidx.push_back((int)j);
             goff.push_back((int)gidx.size());
@@ -932,8 +932,11 @@ static std::unique_ptr<CompatBatch> make_compat_batch(const R *x, const int *n, 
         CUNA_LAUNCH_CHECK();
         // per curve: (1/b, 1/sqrt(pi b), exp(-1/b)) as the single-curve plan
         // forms them, b/4 for the deconvolution, and the output slab
-        std::vector<R> cp(3 * (size_t)nc), qb(nc);
-        std::vector<long> ro(nc);
+        std::vector<R> &cp = G->cpar_h, &qb = G->qb_h;
+        std::vector<long> &ro = G->ro_h;
+        cp.resize(3 * (size_t)nc);
+        qb.resize(nc);
+        ro.resize(nc);
         for (int k = 0; k < nc; ++k) {
             R b, nf;
             int rf;
@@ -944,7 +947,6 @@ static std::unique_ptr<CompatBatch> make_compat_batch(const R *x, const int *n, 
             qb[k] = b * R(0.25);
             ro[k] = (long)cur[k] * B->idist;
         }
-        auto G = std::make_unique<CompatBatch::Group>();
         G->cpar.alloc(sizeof(R) * cp.size());
         G->qb_row.alloc(sizeof(R) * qb.size());
         G->row_off.alloc(sizeof(long) * ro.size());
@@ -959,8 +961,6 @@ static std::unique_ptr<CompatBatch> make_compat_batch(const R *x, const int *n, 
         P->row_off_c = G->row_off.as<long>();
         G->plan = std::move(P);
         B->groups.push_back(std::move(G));
-        // the host tables above are read by the copies in stream order
-        CUNA_HIP(hipStreamSynchronize(s));
     }
     return B;
 }

@@ -107,6 +107,16 @@ __host__ __device__ __forceinline__ cplx<T> mul_i(cplx<T> a) {
 __device__ __forceinline__ float dexp(float v) { return __expf(v); }
 __device__ __forceinline__ double dexp(double v) { return exp(v); }
 
+// Load from memory that no kernel running at the same time writes (plan
+// tables), through the constant address space: with a uniform address the
+// compiler then issues a scalar load (s_load), which waits on its own counter
+// instead of behind the wave's outstanding vector loads.
+template <typename U>
+__device__ __forceinline__ U ldc(const U *p) {
+    using CP = const __attribute__((address_space(4))) U *;
+    return *__builtin_bit_cast(CP, p);
+}
+
 // ----------------------------------------------------------- launch shape --
 // MI355X has 8 XCDs, each with its own L2; consecutive workgroup ids are
 // dispatched round-robin over the XCDs.  Remapping makes neighbouring logical

@@ -57,7 +57,6 @@ struct TiledPoints {
     DevMem rec_orig;  // (anchor, e) by original index
     DevMem order;     // curves by decreasing length (work-queue order)
     DevMem tile_cover;  // per (curve, tile): most points in one window of ntaps anchors
-    int max_cover = 0;  // over all tiles
 
     // src: setup arguments with x/t/mode/compat fields filled
     void build(SetupArgs<T> A, const std::vector<int> &offsets, int G_, int tlog_, int ntaps_,

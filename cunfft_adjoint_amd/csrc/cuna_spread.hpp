@@ -9,9 +9,10 @@
 // workgroup owns one tile: it takes the points of its own bucket plus the
 // boundary classes of its two neighbours and accumulates the taps that land
 // in its tile in LDS, then writes the tile once with coalesced stores.  No
-// global atomics, and both accumulation forms are bitwise reproducible:
-//   k_spread_fx (default): scatter with 64-bit integer atomics in fixed
-//     point (integer sums do not depend on their order);
+// global atomics, and every accumulation form is bitwise reproducible:
+//   k_spread_wv (FP32 default): one wave per tile, 32-bit integer atomics in
+//     fixed point (integer sums do not depend on their order);
+//   k_spread_fx (FP64 default): one workgroup per tile, 64-bit cells;
 //   k_spread (on request, CUNA_PLAN_GATHER): gather, each cell summed by
 //     one thread in a fixed order.
 // The fused bootstrap kernel (k_boot_fused) keeps a whole curve's grid in
@@ -341,6 +342,7 @@ struct SpreadArgs {
     const int *coff;       // curve point offsets (ncurves + 1)
     const T *y;            // values by original index, or null (unit weights)
     long y_slab_stride;    // added per replicate/slab for VAL_PLAIN with several rhs
+    int npts;              // points of all curves (rec and perm hold max(npts, 1))
     T *grid;
     long grid_stride;      // per slab
     int G, tlog, ntiles, Kc, olo, ntaps, nrep;
@@ -357,7 +359,6 @@ struct SpreadArgs {
     // nullable: per (curve, tile), the most points whose anchors fall in one
     // window of ntaps consecutive anchors (no cell receives more weights)
     const int *tile_cover;
-    int fixed32;  // FP32 scatter: every tile's cover is at most kCover32
     // VAL_COMPAT_BOOT (src/gridding.cu:86-117): value y[d] and weight
     // parameter eps_orig[d] of the drawn point d, position of point p
     uint32_t seed;
@@ -467,6 +468,13 @@ struct TileRanges {
     bool tiny;
     template <typename T>
     __device__ __forceinline__ void init(const SpreadArgs<T> &A, int tile, int c, int ntaps) {
+        load(A, tile, c, ntaps);
+        finish();
+    }
+    // issues the bucket loads (b* = first sorted index, n* = one past the
+    // last until finish()); nothing here waits for them
+    template <typename T>
+    __device__ __forceinline__ void load(const SpreadArgs<T> &A, int tile, int c, int ntaps) {
         const int nt = A.ntiles;
         const int *bk = A.bucket + (long)c * (A.Kc + 1);
         const int TT = 1 << A.tlog;
@@ -475,8 +483,8 @@ struct TileRanges {
         tiny = A.wrap1 && A.G < 2 * ntaps;
         S = tiny ? (ntaps + A.G - 1) / A.G + 1 : 0;
         b0 = n0 = o0 = b2 = n2 = o2 = 0;
-        b1 = bk[3 * tile];
-        n1 = bk[3 * tile + 3] - b1;
+        b1 = ldc(bk + 3 * tile);
+        n1 = ldc(bk + 3 * tile + 3);
         if (!tiny) {
             int prev = tile - 1, next = tile + 1;
             bool hp = true, hn = true;
@@ -489,16 +497,22 @@ struct TileRanges {
                 hn = A.periodic;
             }
             if (hp) {
-                b0 = bk[3 * prev + 2];
-                n0 = bk[3 * prev + 3] - b0;
+                b0 = ldc(bk + 3 * prev + 2);
+                n0 = ldc(bk + 3 * prev + 3);
                 o0 = (prev << A.tlog) + TT;
             }
             if (hn) {
-                b2 = bk[3 * next];
-                n2 = bk[3 * next + 1] - b2;
+                b2 = ldc(bk + 3 * next);
+                n2 = ldc(bk + 3 * next + 1);
                 o2 = (next << A.tlog) - TT;
             }
         }
+    }
+    // counts and total from the loaded bounds
+    __device__ __forceinline__ void finish() {
+        n0 -= b0;
+        n1 -= b1;
+        n2 -= b2;
         total = tiny ? n1 * (2 * S + 1) : n0 + n1 + n2;
     }
     // j-th point of the tile: sorted index s and the offset ro (mapped anchor
@@ -679,13 +693,8 @@ __global__ __launch_bounds__(256) void k_tile_cover(SpreadArgs<T> A, int *cover)
     }
 }
 
-// Accumulators of the scatter.  32-bit cells (ds_add_u32, faster than the
-// 64-bit form, and one conversion instruction per weight) hold a round's
-// sums within 2^30: a weight keeps 2^-30 * min(cnt, cover) of the round's
-// largest weight as resolution, which is the float's own 2^-24 or better for
-// a tile whose cover is at most kCover32 points.  Denser tiles, and the FP64
-// library, use 64-bit cells within 2^62.
-constexpr int kCover32 = 64;
+// Accumulators of the scatter: 32-bit cells (ds_add_u32) hold sums within
+// 2^30, 64-bit cells (FP64 library) within 2^62.
 template <typename T>
 struct Fixed64 {
     using U = unsigned long long;
@@ -697,6 +706,46 @@ struct Fixed32 {
     static constexpr int headroom = 30;
     __device__ static U fixed(float x) { return (U)__float2int_rn(x); }
 };
+
+// The NTAPS fixed-point weights of one point, rounded to nearest:
+// n_q = round(v exp(-(q - Q0 - e)^2 bi)), v already carrying the scale.  In
+// FP32 with the table e3 (accurate filters), each tap is one multiply (the
+// step E2^{+-1} outward from the anchor), one fma that also adds the rounding
+// half (with the sign of v: all taps of a point share it) and one truncating
+// conversion.  Otherwise the weights by ratio recurrences (tap_weights_rec,
+// which needs only c1 = exp(-bi)) and FX::fixed per tap.
+template <typename T, int NTAPS, class FX, bool TABLE>
+__device__ __forceinline__ void tap_fixed(T v, T e, T bi, const T *e3, T c1,
+                                          typename FX::U *n) {
+    constexpr int Q0 = (NTAPS - 1) / 2;
+    if constexpr (TABLE && sizeof(T) == 4 && std::is_same<FX, Fixed32>::value) {
+        const float E1 = dexp(-e * e * bi);
+        const float E2 = dexp(2.0f * e * bi);
+        const float E2i = dexp(-2.0f * e * bi);
+        const float h = copysignf(0.5f, v);
+        float up = v * E1;
+#pragma unroll
+        for (int q = Q0; q < NTAPS; ++q) {
+            n[q] = (unsigned)(int)fmaf(up, e3[q], h);
+            up *= E2;
+        }
+        float dn = v * E1 * E2i;
+#pragma unroll
+        for (int q = Q0 - 1; q >= 0; --q) {
+            n[q] = (unsigned)(int)fmaf(dn, e3[q], h);
+            dn *= E2i;
+        }
+    } else {
+        T w[NTAPS];
+        if constexpr (TABLE)
+            tap_weights<T, NTAPS>(v, e, bi, e3, w);
+        else
+            tap_weights_rec<T, NTAPS>(v, e, bi, c1, w);
+#pragma unroll
+        for (int q = 0; q < NTAPS; ++q) n[q] = FX::fixed(w[q]);
+    }
+}
+
 
 // threads per workgroup and point slots per thread of the fixed-point scatter
 constexpr int kSpreadFxThreads = 256, kSpreadFxSlots = 4;
@@ -849,27 +898,198 @@ __device__ __forceinline__ void spread_fx_body(const SpreadArgs<T> &A, T *red, i
     }
 }
 
-// Scatter kernel: W32 (FP32 plans whose tiles all have cover <= kCover32)
-// uses 32-bit cells, else 64-bit.
-template <typename T, int NTAPS, int VM, bool W32>
+// Scatter kernel of the FP64 library: one workgroup per tile, 64-bit cells.
+template <typename T, int NTAPS, int VM>
 __global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A) {
     __shared__ T red[kSpreadFxThreads / 64];
     // no cell of the tile receives more than `cover` weights in a round
     const int c = blockIdx.y / A.nrep;
-    const int cover = A.tile_cover ? A.tile_cover[(long)c * A.ntiles + blockIdx.x]
+    const int cover = A.tile_cover ? ldc(A.tile_cover + (long)c * A.ntiles + blockIdx.x)
                                    : kSpreadFxThreads * kSpreadFxSlots;
-    if constexpr (W32) {
-        static_assert(sizeof(T) == 4, "32-bit cells are for the FP32 library");
-        spread_fx_body<T, NTAPS, VM, Fixed32>(A, red, cover);
-    } else {
-        spread_fx_body<T, NTAPS, VM, Fixed64<T>>(A, red, cover);
-    }
+    spread_fx_body<T, NTAPS, VM, Fixed64<T>>(A, red, cover);
 }
 
 template <typename T>
 __host__ __device__ inline size_t spread_fx_lds_bytes(int tlog, int ntaps) {
     const size_t tt = (size_t)1 << tlog;
     return sizeof(unsigned long long) * (tt + 2 * (size_t)ntaps) + sizeof(T) * tt;
+}
+
+// ------------------------------------------- per-wave fixed-point scatter --
+// Each wave of the tile's workgroup accumulates its own share of the points
+// into its own LDS cells, in fixed point with integer atomics (32-bit cells
+// in FP32, 64-bit in FP64), with its own power-of-two scale:
+//   min(cnt, cover) * vmax * wscale * 2^s <= 2^headroom,
+// cnt = the wave's points this round (<= 64 * kSpreadSlots), vmax their
+// largest |value| (compat filters: largest kept weight / wscale), cover = the
+// tile's most points in one window of ntaps anchors.  No cell can overflow,
+// whatever the density, so no wider fallback and no plan-time read-back is
+// needed; a wave's scale needs no workgroup barrier; and the waves never
+// contend for one address.  After the last round the waves' cells are
+// converted and summed in a fixed wave order, so the grid is bitwise
+// reproducible.  Resolution: 2^-headroom * min(cnt, cover) of the round's
+// largest weight (FP32: 2^-24 for cover <= 64, 2^-22 at worst).  A wave with
+// several rounds folds each into its own float (double) cells.
+constexpr int kSpreadSlots = 4;  // point slots per lane in a round
+
+template <typename T>
+using SpreadFixed = std::conditional_t<sizeof(T) == 4, Fixed32, Fixed64<T>>;
+
+template <typename T>
+__host__ __device__ inline size_t spread_wv_lds_bytes(int tlog, int ntaps, int waves) {
+    const size_t tt = (size_t)1 << tlog;
+    const size_t acc = sizeof(typename SpreadFixed<T>::U) * (tt + 2 * (size_t)ntaps);
+    return (size_t)waves * (acc + sizeof(T) * tt);
+}
+
+template <typename T, int NTAPS, int VM, int WPT>
+__global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
+    using FX = SpreadFixed<T>;
+    using Acc = typename FX::U;
+    using SAcc = std::make_signed_t<Acc>;
+    constexpr int NT = 64 * WPT, PPT = kSpreadSlots, CAP = NT * PPT;
+    constexpr int Q0 = (NTAPS - 1) / 2, HALO = NTAPS;
+    constexpr T kBig = sizeof(T) == 4 ? T(3.0e38f) : T(1.0e300);
+    constexpr int kMaxExp = sizeof(T) == 4 ? 100 : 1000;
+    __shared__ int sexp_sh[WPT];
+    __shared__ int fin_sh[WPT];
+    const int TT = 1 << A.tlog;
+    const int NC = TT + 2 * HALO;  // cells -HALO .. TT+HALO-1
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    Acc *acc_all = reinterpret_cast<Acc *>(cuna_smem);
+    T *facc_all = reinterpret_cast<T *>(acc_all + WPT * NC);
+    Acc *acc = acc_all + wave * NC;
+    T *facc = facc_all + wave * TT;
+
+    const int tile = blockIdx.x;
+    const int slab = blockIdx.y;
+    const int c = slab / A.nrep, rep = slab % A.nrep;
+    TileRanges R;
+    R.init(A, tile, c, NTAPS);
+    const int total = R.total;
+    const bool multi = total > CAP;
+    const int cover = A.tile_cover ? A.tile_cover[(long)c * A.ntiles + tile] : CAP;
+    const T c1 = A.cpar ? A.cpar[3 * c + 2] : A.e3[Q0 + 1];  // exp(-bi)
+    const T bi = A.cpar ? A.cpar[3 * c] : A.bi;
+    const T wscale = A.cpar ? A.cpar[3 * c + 1] : A.wscale;
+    const int tlen = min(TT, A.G - R.tstart);  // cells of the tile inside the grid
+    int ew = 0;
+    (void)frexp(wscale, &ew);
+
+    for (int g = lane; g < NC; g += 64) acc[g] = Acc(0);
+    if (multi)
+        for (int g = lane; g < TT; g += 64) facc[g] = T(0);
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // the wave's clears land before its atomics
+    __builtin_amdgcn_wave_barrier();
+    bool finite = true;
+    int sexp = 0;
+    for (int base = 0; base < total; base += CAP) {
+        const int cnt = min(CAP, total - base);
+        // slot i of the round takes point i (loads coalesce; branch-free, so
+        // all slots issue together); a slot past the end reads point 0
+        int am[PPT], pk[PPT];
+        T ev[PPT], vv[PPT];
+        int wcnt = 0;  // this wave's points in the round
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int i = k * NT + tid;
+            int s, ro;
+            R.get(base + (i < cnt ? i : 0), s, ro);
+            const PointRec<T> pr = A.rec[s];
+            am[k] = pr.a - ro;
+            ev[k] = pr.e;
+            pk[k] = A.perm[s];
+            wcnt += max(0, min(64, cnt - (k * NT + wave * 64)));
+        }
+        T vmax = T(0);
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            T v, e = ev[k];
+            point_value<T, VM>(A, pk[k], c, rep, v, e);
+            const bool ok = k * NT + tid < cnt;
+            vv[k] = ok ? v : T(0);
+            ev[k] = e;
+            T av = fabs(vv[k]);
+            if constexpr (NTAPS % 2 == 1) {
+                // compat filter: e lies up to R_f + 1 cells from the taps and
+                // taps off the grid are dropped, so the largest weight this
+                // tile keeps, at its tap o nearest e, can be far below |v|
+                const int lo = max(-Q0, -am[k]), hi = min(NTAPS - 1 - Q0, tlen - 1 - am[k]);
+                const T o = fmin(fmax(rint(e), T(lo)), T(hi));
+                av = lo <= hi ? av * dexp(-(o - e) * (o - e) * bi) : T(0);
+            }
+            // a non-finite value or position (NaN x gives NaN e) makes the
+            // tile's output NaN
+            if (!(av <= kBig) || (ok && !(fabs(e) <= kBig))) av = T(INFINITY);
+            vmax = fmax(vmax, av);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
+        if (!(vmax <= kBig)) finite = false;
+        const T bound = (T)min(wcnt, cover) * vmax * wscale;
+        int eb = -64;
+        if (bound > T(0) && finite) (void)frexp(bound, &eb);
+        sexp = min(FX::headroom - eb, kMaxExp - ew);
+        const T mul = ldexp(wscale, sexp);
+        if (finite) {
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (k * NT + tid < cnt) {
+                    Acc nq[NTAPS];
+                    // accurate filters (even NTAPS) share one table e3; compat
+                    // batches carry a filter per curve (cpar), so recurrences
+                    tap_fixed<T, NTAPS, FX, NTAPS % 2 == 0>(vv[k] * mul, ev[k], bi, A.e3, c1, nq);
+                    const int c0 = am[k] - Q0;  // first tap's cell
+                    if (R.tiny) {
+#pragma unroll
+                        for (int q = 0; q < NTAPS; ++q) {
+                            const int cell = c0 + q;
+                            if (cell >= -HALO && cell < TT + HALO) atomicAdd(&acc[HALO + cell], nq[q]);
+                        }
+                    } else {
+                        Acc *dst = acc + HALO + c0;
+#pragma unroll
+                        for (int q = 0; q < NTAPS; ++q) atomicAdd(dst + q, nq[q]);
+                    }
+                }
+            }
+        }
+        if (multi) {
+            // fold the round into this wave's float cells; the wave's own
+            // atomics must have landed first (LDS counter drained)
+            __builtin_amdgcn_s_waitcnt(0xc07f);
+            __builtin_amdgcn_wave_barrier();
+            const T inv = ldexp(T(1), -sexp);
+            for (int g = lane; g < TT; g += 64) {
+                facc[g] += (T)(SAcc)acc[HALO + g] * inv;
+                acc[HALO + g] = Acc(0);
+            }
+            for (int h = lane; h < 2 * HALO; h += 64) acc[h < HALO ? h : TT + h] = Acc(0);
+            __builtin_amdgcn_s_waitcnt(0xc07f);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (lane == 0) {
+        sexp_sh[wave] = sexp;
+        fin_sh[wave] = finite ? 1 : 0;
+    }
+    // every wave's cells and scale are read below by all threads
+    __syncthreads();
+    bool all_finite = true;
+    T inv[WPT];
+#pragma unroll
+    for (int w = 0; w < WPT; ++w) {
+        all_finite = all_finite && fin_sh[w] != 0;
+        inv[w] = ldexp(T(1), -sexp_sh[w]);
+    }
+    T *gout = A.grid + (long)slab * A.grid_stride + R.tstart;
+    for (int g = tid; g < tlen; g += NT) {
+        T v = T(0);
+#pragma unroll
+        for (int w = 0; w < WPT; ++w)
+            v += multi ? facc_all[w * TT + g] : (T)(SAcc)acc_all[w * NC + HALO + g] * inv[w];
+        gout[g] = all_finite ? v : T(NAN);
+    }
 }
 
 // ------------------------------------------- fused batched bootstrap --

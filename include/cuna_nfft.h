@@ -62,8 +62,9 @@ void cuna_nfft_default_params(cuna_nfft_params *p);
 
 /* n points x (accurate: any real, reduced mod 1; compat: [-1/2, 1/2]).
  * accurate: M modes, M a power of two >= 2; compat: M = ng outputs.
- * Plan creation (every form) waits once for `stream`: it reads back the
- * binning's largest per-tile point count, which picks the accumulator. */
+ * Plan creation (every form) is ordered on `stream` and does not wait for
+ * it (the first plan of a grid size uploads that size's FFT twiddle tables
+ * with a synchronous copy). */
 int cuna_nfft_plan_create(cuna_nfft_plan *plan, const dTyp *x, int n, int M,
                           const cuna_nfft_params *params, hipStream_t stream);
 

@@ -305,6 +305,49 @@ def test_pinned_filter_properties_single(prec):
 
 
 @pytest.mark.parametrize("prec", ["f", "d"])
+def test_pinned_filter_properties_does_not_wait(prec):
+    """generate_pinned_filter_properties is ordered on its stream and does not
+    wait for it, as src/gaussian_filter.cu:43-66 is: enqueued behind a bounded
+    spin kernel on the same stream, it returns while that stream is still busy
+    (hipStreamQuery: not ready), and the transform ordered after it matches
+    the oracle."""
+    import time
+    rng = np.random.default_rng(34)
+    n, ng = 3000, 1 << 12
+    x = rng.uniform(-0.5, 0.5, n).astype(NP[prec])
+    y = rng.standard_normal(n).astype(NP[prec])
+    xd, yd = dev(x), dev(y)
+    L = cu.load(prec)
+    _, _, FP, _ = cu._lib.make_types(prec)
+    rf = L.cuna_filter_radius()
+    tdt = torch.float32 if prec == "f" else torch.float64
+    warm = cu.generate_filter_properties(xd, n, ng)  # this size's FFT tables exist now
+    warm.free()
+    E1, E2, E3 = (torch.empty(k, dtype=tdt, device="cuda") for k in (n, n, rf))
+    h = FP(0, 0, 0, E1.data_ptr(), E2.data_ptr(), E3.data_ptr())
+    d = _hip_malloc(ctypes.sizeof(FP))
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipStreamQuery.argtypes = [ctypes.c_void_p]
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        torch.cuda._sleep(200_000_000)  # bounded spin, ~0.1 s
+    t0 = time.perf_counter()
+    cu.generate_pinned_filter_properties(xd, n, ng, h, d, stream=s)
+    elapsed = time.perf_counter() - t0
+    busy = hip.hipStreamQuery(ctypes.c_void_p(s.cuda_stream))
+    assert busy == 600, (busy, elapsed)  # hipErrorNotReady: the spin still runs
+    fh = torch.zeros(ng, dtype=torch.complex64 if prec == "f" else torch.complex128,
+                     device="cuda")
+    cu.cunfft_adjoint_raw_async(xd, yd, fh, n, ng, d, stream=s)
+    s.synchronize()
+    assert h.b == pytest.approx(oracle.compat_params(n, ng, prec == "d")[0],
+                                rel=1e-7 if prec == "f" else 1e-15)
+    assert rel_l2(host(fh), oracle.compat_adjoint(x, y, ng, prec == "d")) < TOL[prec]
+    L.free_filter_properties(d, None)
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
 def test_compat_batched_rejects_bad_sizes(prec):
     x = dev(np.zeros(10, dtype=NP[prec]))
     fh = torch.zeros(64, dtype=torch.complex64 if prec == "f" else torch.complex128,
