@@ -246,7 +246,7 @@ def bench_c2(args, torch, cu):
     G = info.G
     bytes_total = N * (4 + 4) + 2 * G * 4 + M * 8
     kernels = {
-        "spread (k_spread_fx)": (spread, N * (4 + 4) + G * 4, "k_spread_fx"),
+        "spread (k_spread_wv)": (spread, N * (4 + 4) + G * 4, "k_spread_wv"),
         "fft pass 1 (k_fft_col)": (colp, G * 4 + (G // 2) * 8, "k_fft_col"),
         "fft pass 2 + deconvolution (k_fft_row_real)": (rowp, (G // 2) * 8 + M * 8,
                                                          "k_fft_row_real"),
@@ -286,7 +286,15 @@ def bench_c2(args, torch, cu):
     }
 
 
-def bench_bootstrap(args, torch, cu):
+def hbm_roofline(nbytes, ms, copy_bw):
+    """Compulsory bytes over a measured time, against the HBM peak and the
+    box's own measured device-to-device copy bandwidth."""
+    gbps = nbytes / (ms * 1e-3)
+    return {"bound": "hbm", "compulsory_bytes": nbytes, "ms": ms, "achieved_GBps": gbps / 1e9,
+            "frac_of_hbm_peak": gbps / HBM_PEAK, "frac_of_copy_bw": gbps / copy_bw}
+
+
+def bench_bootstrap(args, torch, cu, copy_bw):
     """Batched bootstrap (the batching configuration): 1024 light curves of
     log-uniform length in [100, 5e4] (8.2e6 points), times in days, FP64 phase
     reduction with df = 1/3650 c/d, M = 2^14, 100 replicates per curve, two
@@ -387,6 +395,12 @@ def bench_bootstrap(args, torch, cu):
     plan_b.close()
     del out_y, out_w
     torch.cuda.empty_cache()
+    npts = int(offs[-1])
+    spectra = ncurves * nbs * M * 8  # one complex64 spectrum of M modes per replicate
+    # compulsory: read t (FP64) and y (FP32) once, write out_y and out_w
+    boot_bytes = npts * (8 + 4) + 2 * spectra
+    # Lomb-Scargle: read t and y once, write M/2 FP32 powers + peak per replicate
+    ls_bytes = npts * (8 + 4) + ncurves * nbs * (M // 2 * 4 + 8)
     return {
         "workload": f"{ncurves} light curves, lengths log-uniform [100, 5e4] "
                     f"({int(offs[-1])} points), M=2^14, {nbs} replicates x 2 transforms "
@@ -399,6 +413,9 @@ def bench_bootstrap(args, torch, cu):
         "max_rel_err_vs_ndft": max(errs),
         "batched_vs_plain_rel_diff": agree,
         "transforms_per_s_batched": ncurves * nbs * 2 / (ms_b * 1e-3),
+        "roofline": dict(hbm_roofline(boot_bytes, ms_b, copy_bw),
+                         what="batched fused launch: compulsory bytes = points (t FP64 + y "
+                              "FP32) read once + out_y + out_w written"),
         "lomb_scargle": {
             "what": "fused periodograms of the same replicates: power at k = 0..M/2-1 "
                     "(+ peak), or the peak only; no spectra written",
@@ -408,6 +425,9 @@ def bench_bootstrap(args, torch, cu):
             "ms_power": ms_ls,
             "ms_peak_only": ms_pk,
             "max_abs_err_vs_fp64_periodogram": ls_err,
+            "roofline": dict(hbm_roofline(ls_bytes, ms_ls, copy_bw),
+                             what="compulsory bytes = points read once + M/2 powers and the "
+                                  "peak written per replicate"),
         },
     }
 
@@ -726,8 +746,10 @@ def main(argv=None):
         "copy_bw_GBps": bw / 1e9,
         "cpu_baseline": None,
     }
+    res["roofline"]["frac_of_copy_bw"] = res["roofline"]["achieved"] * 1e9 / bw
+    res["pipeline"]["frac_of_copy_bw"] = res["pipeline"]["achieved_GBps"] * 1e9 / bw
     if not args.no_bootstrap:
-        res["bootstrap"] = bench_bootstrap(args, torch, cu)
+        res["bootstrap"] = bench_bootstrap(args, torch, cu, bw)
     if not args.no_extras:
         cu.load("d")
         extras = {

@@ -110,25 +110,24 @@ __device__ __forceinline__ void dft(cplx<T> *v) {
 }
 
 // ------------------------------------------------------------ radix plan --
-// log2 of the radix of pass p for a length-2^lg transform: radix 16 passes,
-// with the remainder taken by one radix-8/4 pass (or 8+4 for lg = 4k+1).
-constexpr int num_passes(int lg) {
+// log2 of the radix of pass p for a length-2^lg transform with radices up to
+// 2^mr (mr = 4: radix 16, mr = 3: radix 8): full-radix passes, with the
+// remainder taken by one smaller pass, or, for lg = k mr + 1, by two
+// (radix 8 and 4 after radix-16 passes, 4 and 4 after radix-8 passes).
+constexpr int num_passes(int lg, int mr = 4) {
     if (lg <= 0) return 0;
-    if (lg == 1) return 1;
-    int n16 = lg / 4, rem = lg % 4;
-    if (rem == 0) return n16;
-    if (rem == 1) return n16 + 1;  // (n16-1) x16, then 8, 4
-    return n16 + 1;
+    if (lg <= mr) return 1;
+    return lg % mr == 0 ? lg / mr : lg / mr + 1;
 }
-constexpr int pass_log2(int lg, int p) {
-    if (lg == 1) return 1;
-    int n16 = lg / 4, rem = lg % 4;
-    if (rem == 0) return 4;
+constexpr int pass_log2(int lg, int p, int mr = 4) {
+    if (lg <= mr) return lg;
+    const int n = lg / mr, rem = lg % mr;
+    if (rem == 0) return mr;
     if (rem == 1) {
-        if (p < n16 - 1) return 4;
-        return p == n16 - 1 ? 3 : 2;
+        if (p < n - 1) return mr;
+        return p == n - 1 ? (mr + 2) / 2 : (mr + 1) / 2;
     }
-    return p < n16 ? 4 : rem;
+    return p < n ? mr : rem;
 }
 
 // ------------------------------------------------------------ LDS layout --
@@ -200,12 +199,12 @@ struct NoHook {
 // LDS (used to stage a twiddle table whose loads were issued earlier).
 // `tw` is a table pointer or any object with operator[](int) giving w_L^e.
 template <typename T, int S, int LG, int W, int NT, int P, int LOGNS, class Loader,
-          bool INPLACE = false, class Hook = NoHook, class TW = const cplx<T> *>
+          bool INPLACE = false, class Hook = NoHook, class TW = const cplx<T> *, int MR = 4>
 __device__ __forceinline__ void fft_pass(cplx<T> *lds, TW tw, const Loader &ld,
                                          const Hook &hook = Hook()) {
-    constexpr int NP = num_passes(LG);
+    constexpr int NP = num_passes(LG, MR);
     if constexpr (P < NP) {
-        constexpr int LR = pass_log2(LG, P);
+        constexpr int LR = pass_log2(LG, P, MR);
         constexpr int R = 1 << LR;
         constexpr int L = 1 << LG;
         constexpr int Q = L >> LR;  // butterflies per sequence
@@ -258,14 +257,17 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, TW tw, const Loader &ld,
             }
         }
         __syncthreads();
-        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader, INPLACE, Hook, TW>(lds, tw, ld, hook);
+        fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader, INPLACE, Hook, TW, MR>(lds, tw, ld,
+                                                                                   hook);
     }
 }
 
-template <typename T, int S, int LG, int W, int NT, class Loader, class Hook = NoHook>
+// MR: log2 of the largest radix (4: 16 values per butterfly; 3: 8, so that
+// twice the threads share the work)
+template <typename T, int S, int LG, int W, int NT, int MR = 4, class Loader, class Hook = NoHook>
 __device__ __forceinline__ void lds_fft(cplx<T> *lds, const cplx<T> *__restrict__ tw,
                                         const Loader &ld, const Hook &hook = Hook()) {
-    fft_pass<T, S, LG, W, NT, 0, 0, Loader, false, Hook>(lds, tw, ld, hook);
+    fft_pass<T, S, LG, W, NT, 0, 0, Loader, false, Hook, const cplx<T> *, MR>(lds, tw, ld, hook);
 }
 
 // A twiddle table of L entries copied into LDS: load() issues the global
@@ -567,13 +569,14 @@ __global__ __launch_bounds__(NT) void k_fft_small(Loader ld, Sink sk, cplx<T> *p
 // Pass 1 of the two-pass FFT: W consecutive columns n1 of the L1 x L2 view
 // (n = n1 + L1 n2) per workgroup; length-L2 FFT over n2, multiply by
 // w_N^(S n1 r2), store at n1 + L1 r2 of dst (may alias the source).
-template <typename T, int S, int LG2, int W, int NT, class Loader>
+template <typename T, int S, int LG2, int W, int NT, class Loader, int MR = 4>
 __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long dst_stride, int L1,
                                                 const cplx<T> *tw, Tw2<T> twN) {
     constexpr int L2 = 1 << LG2;
     constexpr bool TAB = Loader::kStage && lds_table(LG2);
-    // epilogue twiddles held through the FFT: only where registers are ample
-    constexpr bool PRE = Loader::kStage && NT <= 256 && sizeof(T) == 4;
+    // epilogue twiddles held through the FFT only where registers are ample:
+    // up to 256 threads with radix 16, 128 with radix 8
+    constexpr bool PRE = Loader::kStage && (NT << (4 - MR)) <= 256 && sizeof(T) == 4;
     constexpr int OPT = PRE ? (W * L2 + NT - 1) / NT : 1;
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
     cplx<T> *twl = lds + W * L2;
@@ -598,9 +601,9 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
     auto fetch = [&](int e, int w) __attribute__((always_inline)) { return lb.at(n10 + w + (long)L1 * e); };
     if constexpr (TAB) {
         auto hook = [&]() __attribute__((always_inline)) { tab.store(twl, tid); };
-        lds_fft<T, S, LG2, W, NT>(lds, twl, fetch, hook);
+        lds_fft<T, S, LG2, W, NT, MR>(lds, twl, fetch, hook);
     } else {
-        lds_fft<T, S, LG2, W, NT>(lds, tw, fetch);
+        lds_fft<T, S, LG2, W, NT, MR>(lds, tw, fetch);
     }
     cplx<T> *o = dst + b * dst_stride;
     if constexpr (PRE) {
@@ -628,7 +631,7 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
 // mirrors (L2 - rho) mod L2 (row 0's mirror slot holds row L2/2; both are
 // self-paired).  Length-L1 FFTs over n1, then pairs (Z[r], Z[N-r]) with
 // r = rho + L2 r1 go through emit_pair into the sink.
-template <typename T, int LG1, int H, int NT, class Sink>
+template <typename T, int LG1, int H, int NT, class Sink, int MR = 4>
 __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long src_stride,
                                                      int L2, Sink sk, const cplx<T> *tw,
                                                      Tw2<T> twG) {
@@ -664,9 +667,9 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
     auto fetch = [&](int e, int s) __attribute__((always_inline)) { return z[(long)row_of(s) * L1 + e]; };
     if constexpr (TAB) {
         auto hook = [&]() __attribute__((always_inline)) { tab.store(twl, tid); };
-        lds_fft<T, 1, LG1, W, NT>(lds, twl, fetch, hook);
+        lds_fft<T, 1, LG1, W, NT, MR>(lds, twl, fetch, hook);
     } else {
-        lds_fft<T, 1, LG1, W, NT>(lds, tw, fetch);
+        lds_fft<T, 1, LG1, W, NT, MR>(lds, tw, fetch);
     }
     const auto skb = sk.for_row(b);
     cplx<T> *o = sk.row_out(b);
