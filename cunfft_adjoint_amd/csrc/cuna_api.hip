@@ -209,6 +209,9 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
         const size_t ncov = (size_t)std::max(ncurves, 1) * std::max(ntiles, 1);
         tile_cover.alloc(sizeof(int) * ncov);
         launch_tile_cover<T>(C, ncurves, tile_cover.as<int>(), s);
+        tile_desc.alloc(sizeof(TileDesc) * ncov);
+        launch_tile_order(bucket.as<int>(), Kc, ntiles, ncurves, periodic, tile_cover.as<int>(),
+                          tile_desc.as<TileDesc>(), s);
     }
     // work-queue order of the curves for the fused batched kernels
     std::vector<int> ord(ncurves);
@@ -250,6 +253,7 @@ SpreadArgs<T> Plan<T>::spread_args(const TiledPoints<T> &P) const {
     A.wscale = wscale;
     A.cpar = cpar_c;
     A.tile_cover = P.tile_cover.template as<int>();
+    A.tile_desc = P.tile_desc.template as<TileDesc>();
     for (int q = 0; q < kMaxTaps; ++q) A.e3[q] = e3[q];
     return A;
 }

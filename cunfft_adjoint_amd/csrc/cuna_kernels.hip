@@ -495,6 +495,19 @@ void launch_tile_cover(const SpreadArgs<T> &A, int ncurves, int *cover, hipStrea
     CUNA_LAUNCH_CHECK();
 }
 
+void launch_tile_order(const int *bucket, int Kc, int ntiles, int ncurves, bool periodic,
+                       const int *cover, TileDesc *desc, hipStream_t s) {
+    if (ncurves <= 0 || ntiles <= 0) return;
+    int P2 = 1;
+    while (P2 < ntiles) P2 <<= 1;
+    require(P2 <= 8192, "too many spread tiles for the work order", CUNA_ERROR_UNSUPPORTED);
+    const size_t lds = 2 * sizeof(int) * (size_t)P2;
+    set_lds(k_tile_order, lds);
+    hipLaunchKernelGGL(k_tile_order, dim3(ncurves), dim3(kOrderThreads), lds, s, bucket, Kc,
+                       ntiles, P2, periodic ? 1 : 0, cover, desc);
+    CUNA_LAUNCH_CHECK();
+}
+
 template <typename T>
 void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s) {
     require(ntaps % 2 == 0, "unsupported filter width", CUNA_ERROR_UNSUPPORTED);

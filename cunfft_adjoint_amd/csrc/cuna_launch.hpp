@@ -77,6 +77,9 @@ void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s)
 // per (curve, tile) point cover of the spread's scale (A: rec, bucket, tiles)
 template <typename T>
 void launch_tile_cover(const SpreadArgs<T> &A, int ncurves, int *cover, hipStream_t s);
+// per curve, the scatter's work order (tiles by decreasing point count)
+void launch_tile_order(const int *bucket, int Kc, int ntiles, int ncurves, bool periodic,
+                       const int *cover, TileDesc *desc, hipStream_t s);
 
 // fused batched bootstrap (grid in LDS): false when this size/filter has no
 // fused kernel

@@ -57,6 +57,7 @@ struct TiledPoints {
     DevMem rec_orig;  // (anchor, e) by original index
     DevMem order;     // curves by decreasing length (work-queue order)
     DevMem tile_cover;  // per (curve, tile): most points in one window of ntaps anchors
+    DevMem tile_desc;   // per curve: the scatter's work order (TileDesc per tile)
 
     // src: setup arguments with x/t/mode/compat fields filled
     void build(SetupArgs<T> A, const std::vector<int> &offsets, int G_, int tlog_, int ntaps_,

@@ -334,6 +334,29 @@ static __global__ __launch_bounds__(kScanNT) void k_scan_apply(const int *in, lo
 // ----------------------------------------------------------- spread --
 enum ValueMode { VAL_PLAIN = 0, VAL_COMPAT_BOOT = 1, VAL_COUNTS = 2 };
 
+// One entry of a curve's spread work order: the tile, its three sorted
+// point ranges [b, e) (empty where a neighbour is absent: non-periodic grid
+// edges) and its cover.  Built once per plan with the tiles in decreasing
+// order of their own point counts (ties by tile index), so that the densest
+// tiles start first and the launch ends on light ones.
+struct TileDesc {
+    int tile, b0, e0, b1, e1, b2, e2, cover;
+};
+// scalar loads of one entry (merged into one 32-byte s_load)
+__device__ __forceinline__ TileDesc ldc_desc(const TileDesc *p) {
+    const int *q = reinterpret_cast<const int *>(p);
+    TileDesc d;
+    d.tile = ldc(q);
+    d.b0 = ldc(q + 1);
+    d.e0 = ldc(q + 2);
+    d.b1 = ldc(q + 3);
+    d.e1 = ldc(q + 4);
+    d.b2 = ldc(q + 5);
+    d.e2 = ldc(q + 6);
+    d.cover = ldc(q + 7);
+    return d;
+}
+
 template <typename T>
 struct SpreadArgs {
     const PointRec<T> *rec;  // points sorted by anchor within each curve
@@ -359,6 +382,9 @@ struct SpreadArgs {
     // nullable: per (curve, tile), the most points whose anchors fall in one
     // window of ntaps consecutive anchors (no cell receives more weights)
     const int *tile_cover;
+    // nullable: per curve, ntiles work-order entries (the scatter kernels
+    // take entry blockIdx.x of their curve instead of tile blockIdx.x)
+    const TileDesc *tile_desc;
     // VAL_COMPAT_BOOT (src/gridding.cu:86-117): value y[d] and weight
     // parameter eps_orig[d] of the drawn point d, position of point p
     uint32_t seed;
@@ -508,12 +534,40 @@ struct TileRanges {
             }
         }
     }
+    // the same ranges from a work-order entry (one scalar load of 32 bytes
+    // in place of the bucket loads)
+    template <typename T>
+    __device__ __forceinline__ void from_desc(const SpreadArgs<T> &A, const TileDesc &d,
+                                              int ntaps) {
+        const int nt = A.ntiles;
+        const int TT = 1 << A.tlog;
+        G = A.G;
+        tstart = d.tile << A.tlog;
+        tiny = A.wrap1 && A.G < 2 * ntaps;
+        S = tiny ? (ntaps + A.G - 1) / A.G + 1 : 0;
+        b0 = d.b0;
+        n0 = d.e0;
+        b1 = d.b1;
+        n1 = d.e1;
+        b2 = d.b2;
+        n2 = d.e2;
+        const int prev = d.tile > 0 ? d.tile - 1 : nt - 1;
+        const int next = d.tile + 1 < nt ? d.tile + 1 : 0;
+        o0 = (prev << A.tlog) + TT;
+        o2 = (next << A.tlog) - TT;
+        finish();
+    }
     // counts and total from the loaded bounds
     __device__ __forceinline__ void finish() {
         n0 -= b0;
         n1 -= b1;
         n2 -= b2;
         total = tiny ? n1 * (2 * S + 1) : n0 + n1 + n2;
+    }
+    // an interior tile: its three ranges are adjacent buckets (keys 3t-1 ..
+    // 3t+3) and share the offset tstart, so point j is sorted index b0 + j
+    __device__ __forceinline__ bool contiguous() const {
+        return !tiny && b0 + n0 == b1 && b1 + n1 == b2 && o0 == tstart && o2 == tstart;
     }
     // j-th point of the tile: sorted index s and the offset ro (mapped anchor
     // = a - ro); selects rather than branches, so that loads indexed by it
@@ -693,6 +747,69 @@ __global__ __launch_bounds__(256) void k_tile_cover(SpreadArgs<T> A, int *cover)
     }
 }
 
+// Work order of the scatter kernels, one workgroup per curve: the curve's
+// tiles sorted by decreasing own point count (bitonic sort in LDS of P2 >=
+// ntiles keys, P2 a power of two <= 8192), then one TileDesc per position.
+constexpr int kOrderThreads = 1024;
+static __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int *bucket, int Kc,
+                                                                     int ntiles, int P2,
+                                                                     int periodic,
+                                                                     const int *cover,
+                                                                     TileDesc *desc) {
+    int *key = reinterpret_cast<int *>(cuna_smem);
+    int *val = key + P2;
+    const int c = blockIdx.x;
+    const int *bk = bucket + (long)c * (Kc + 1);
+    for (int i = threadIdx.x; i < P2; i += kOrderThreads) {
+        key[i] = i < ntiles ? bk[3 * i + 3] - bk[3 * i] : -1;
+        val[i] = i;
+    }
+    __syncthreads();
+    // "before": more points, or as many and a smaller tile index
+    auto before = [&](int a, int b) { return key[a] > key[b] || (key[a] == key[b] && val[a] < val[b]); };
+    for (int k = 2; k <= P2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P2; i += kOrderThreads) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    if (up ? before(l, i) : before(i, l)) {
+                        const int tk = key[i], tv = val[i];
+                        key[i] = key[l];
+                        val[i] = val[l];
+                        key[l] = tk;
+                        val[l] = tv;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < ntiles; i += kOrderThreads) {
+        const int t = val[i];
+        TileDesc d;
+        d.tile = t;
+        d.b1 = bk[3 * t];
+        d.e1 = bk[3 * t + 3];
+        int prev = t - 1, next = t + 1;
+        bool hp = true, hn = true;
+        if (prev < 0) {
+            prev = ntiles - 1;
+            hp = periodic != 0;
+        }
+        if (next >= ntiles) {
+            next = 0;
+            hn = periodic != 0;
+        }
+        d.b0 = hp ? bk[3 * prev + 2] : 0;
+        d.e0 = hp ? bk[3 * prev + 3] : 0;
+        d.b2 = hn ? bk[3 * next] : 0;
+        d.e2 = hn ? bk[3 * next + 1] : 0;
+        d.cover = cover ? cover[(long)c * ntiles + t] : 0x7fffffff;
+        desc[(long)c * ntiles + i] = d;
+    }
+}
+
 // Accumulators of the scatter: 32-bit cells (ds_add_u32) hold sums within
 // 2^30, 64-bit cells (FP64 library) within 2^62.
 template <typename T>
@@ -766,13 +883,15 @@ __device__ __forceinline__ void spread_fx_body(const SpreadArgs<T> &A, T *red, i
     // 2^sexp also keeps wscale * 2^sexp finite
     constexpr int kMaxExp = sizeof(T) == 4 ? 100 : 1000;
 
-    // consecutive tiles go to different XCDs (hardware round-robin): dense
-    // regions of clustered points are spread over all of them
-    const int tile = blockIdx.x;
+    // consecutive workgroups go to different XCDs (hardware round-robin):
+    // the densest tiles, first in the work order, are spread over all of them
     const int slab = blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
     TileRanges R;
-    R.init(A, tile, c, NTAPS);
+    if (A.tile_desc)
+        R.from_desc(A, ldc_desc(A.tile_desc + (long)c * A.ntiles + blockIdx.x), NTAPS);
+    else
+        R.init(A, blockIdx.x, c, NTAPS);
     const int total = R.total;
     const bool multi = total > CAP;
     const int tid = threadIdx.x;
@@ -904,8 +1023,10 @@ __global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A)
     __shared__ T red[kSpreadFxThreads / 64];
     // no cell of the tile receives more than `cover` weights in a round
     const int c = blockIdx.y / A.nrep;
-    const int cover = A.tile_cover ? ldc(A.tile_cover + (long)c * A.ntiles + blockIdx.x)
-                                   : kSpreadFxThreads * kSpreadFxSlots;
+    const long e = (long)c * A.ntiles + blockIdx.x;
+    const int cover = A.tile_desc    ? ldc(&A.tile_desc[e].cover)
+                      : A.tile_cover ? ldc(A.tile_cover + e)
+                                     : kSpreadFxThreads * kSpreadFxSlots;
     spread_fx_body<T, NTAPS, VM, Fixed64<T>>(A, red, cover);
 }
 
@@ -961,14 +1082,20 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
     Acc *acc = acc_all + wave * NC;
     T *facc = facc_all + wave * TT;
 
-    const int tile = blockIdx.x;
     const int slab = blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
     TileRanges R;
-    R.init(A, tile, c, NTAPS);
+    int cover = CAP;
+    if (A.tile_desc) {
+        const TileDesc d = ldc_desc(A.tile_desc + (long)c * A.ntiles + blockIdx.x);
+        R.from_desc(A, d, NTAPS);
+        cover = d.cover;
+    } else {
+        R.init(A, blockIdx.x, c, NTAPS);
+        if (A.tile_cover) cover = A.tile_cover[(long)c * A.ntiles + blockIdx.x];
+    }
     const int total = R.total;
     const bool multi = total > CAP;
-    const int cover = A.tile_cover ? A.tile_cover[(long)c * A.ntiles + tile] : CAP;
     const T c1 = A.cpar ? A.cpar[3 * c + 2] : A.e3[Q0 + 1];  // exp(-bi)
     const T bi = A.cpar ? A.cpar[3 * c] : A.bi;
     const T wscale = A.cpar ? A.cpar[3 * c + 1] : A.wscale;
@@ -983,6 +1110,7 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
     __builtin_amdgcn_wave_barrier();
     bool finite = true;
     int sexp = 0;
+    const bool contig = R.contiguous();  // uniform
     for (int base = 0; base < total; base += CAP) {
         const int cnt = min(CAP, total - base);
         // slot i of the round takes point i (loads coalesce; branch-free, so
@@ -993,8 +1121,14 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const int i = k * NT + tid;
+            const int j = base + (i < cnt ? i : 0);
             int s, ro;
-            R.get(base + (i < cnt ? i : 0), s, ro);
+            if (contig) {
+                s = R.b0 + j;
+                ro = R.tstart;
+            } else {
+                R.get(j, s, ro);
+            }
             const PointRec<T> pr = A.rec[s];
             am[k] = pr.a - ro;
             ev[k] = pr.e;
