@@ -811,16 +811,35 @@ static __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int *
 }
 
 // Accumulators of the scatter: 32-bit cells (ds_add_u32) hold sums within
-// 2^30, 64-bit cells (FP64 library) within 2^62.
+// 2^30, 64-bit cells (FP64 library) within 2^62.  In FP64 each weight is also
+// kept within 2^`single` = 2^50 (a second bound on the scale) and rounded to
+// the nearest integer by adding 1.5 * 2^52: for |x| < 2^51 the sum lies in
+// [2^52, 2^53), whose spacing is 1, so its low mantissa bits are round(x) and
+//   round(x) = bits(x + 1.5 * 2^52) - bits(1.5 * 2^52)
+// as 64-bit integers (one add, or one fma with the weight's product, and one
+// 64-bit subtract per tap).
 template <typename T>
 struct Fixed64 {
     using U = unsigned long long;
     static constexpr int headroom = 62;
-    __device__ static U fixed(T x) { return to_fixed64(x); }
+    static constexpr int single = sizeof(T) == 8 ? 50 : 0;  // 0: no per-weight bound
+    static constexpr double kMagic = 0x1.8p52;
+    static constexpr U kMagicBits = 0x4338000000000000ull;
+    __device__ static U fixed(T x) {
+        if constexpr (sizeof(T) == 8)
+            return (U)__double_as_longlong(x + kMagic) - kMagicBits;
+        else
+            return to_fixed64(x);
+    }
+    // round(a * b), FP64 only
+    __device__ static U fixed_fma(double a, double b) {
+        return (U)__double_as_longlong(fma(a, b, kMagic)) - kMagicBits;
+    }
 };
 struct Fixed32 {
     using U = unsigned;
     static constexpr int headroom = 30;
+    static constexpr int single = 0;
     __device__ static U fixed(float x) { return (U)__float2int_rn(x); }
 };
 
@@ -850,6 +869,23 @@ __device__ __forceinline__ void tap_fixed(T v, T e, T bi, const T *e3, T c1,
 #pragma unroll
         for (int q = Q0 - 1; q >= 0; --q) {
             n[q] = (unsigned)(int)fmaf(dn, e3[q], h);
+            dn *= E2i;
+        }
+    } else if constexpr (TABLE && sizeof(T) == 8 && std::is_same<FX, Fixed64<T>>::value) {
+        // the same walk in FP64, each tap rounded by FX::fixed_fma
+        const double E1 = dexp(-e * e * bi);
+        const double E2 = dexp(2.0 * e * bi);
+        const double E2i = dexp(-2.0 * e * bi);
+        double up = v * E1;
+#pragma unroll
+        for (int q = Q0; q < NTAPS; ++q) {
+            n[q] = FX::fixed_fma(up, e3[q]);
+            up *= E2;
+        }
+        double dn = v * E1 * E2i;
+#pragma unroll
+        for (int q = Q0 - 1; q >= 0; --q) {
+            n[q] = FX::fixed_fma(dn, e3[q]);
             dn *= E2i;
         }
     } else {
@@ -972,25 +1008,33 @@ __device__ __forceinline__ void spread_fx_body(const SpreadArgs<T> &A, T *red, i
         int ew = 0;
         (void)frexp(wscale, &ew);
         sexp = min(FX::headroom - eb, kMaxExp - ew);
+        if constexpr (FX::single > 0) {
+            // and every weight on the tile's cells within 2^single
+            int e1 = -64;
+            const T b1 = vmax * wscale;
+            if (b1 > T(0) && finite) (void)frexp(b1, &e1);
+            sexp = min(sexp, FX::single - e1);
+        }
         const T mul = ldexp(wscale, sexp);
         if (finite) {
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
                 if (base + k * NT + tid < base + cnt) {
-                    T w[NTAPS];
-                    tap_weights_rec<T, NTAPS>(vv[k] * mul, ev[k], bi, c1, w);
+                    Acc nq[NTAPS];
+                    // accurate filters (even NTAPS) share one table e3; compat
+                    // batches carry a filter per curve (cpar), so recurrences
+                    tap_fixed<T, NTAPS, FX, NTAPS % 2 == 0>(vv[k] * mul, ev[k], bi, A.e3, c1, nq);
                     const int c0 = am[k] - Q0;  // first tap's cell
                     if (R.tiny) {
 #pragma unroll
                         for (int q = 0; q < NTAPS; ++q) {
                             const int cell = c0 + q;
-                            if (cell >= -HALO && cell < TT + HALO)
-                                atomicAdd(&acc[HALO + cell], FX::fixed(w[q]));
+                            if (cell >= -HALO && cell < TT + HALO) atomicAdd(&acc[HALO + cell], nq[q]);
                         }
                     } else {
                         Acc *dst = acc + HALO + c0;
 #pragma unroll
-                        for (int q = 0; q < NTAPS; ++q) atomicAdd(dst + q, FX::fixed(w[q]));
+                        for (int q = 0; q < NTAPS; ++q) atomicAdd(dst + q, nq[q]);
                     }
                 }
             }
@@ -1164,6 +1208,12 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
         int eb = -64;
         if (bound > T(0) && finite) (void)frexp(bound, &eb);
         sexp = min(FX::headroom - eb, kMaxExp - ew);
+        if constexpr (FX::single > 0) {
+            int e1 = -64;
+            const T b1 = vmax * wscale;
+            if (b1 > T(0) && finite) (void)frexp(b1, &e1);
+            sexp = min(sexp, FX::single - e1);
+        }
         const T mul = ldexp(wscale, sexp);
         if (finite) {
 #pragma unroll
