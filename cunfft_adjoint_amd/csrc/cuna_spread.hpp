@@ -1436,7 +1436,12 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
     twG.lo = tl + NLN + NHN;
     twG.hi = tl + NLN + NHN + NLG;
     twG.lb = LBG;
-    __shared__ int item_sh, overflow_sh;
+    __shared__ int item_sh;
+    __shared__ int dmax_sh[NT / 64];
+    // draws per block of 32 anchors (at most G/32 = 1024 blocks)
+    constexpr int NB = G / 32;
+    static_assert(NB >= 2 && NTAPS <= 32, "a tap window spans at most two anchor blocks");
+    __shared__ int hist_sh[NB];
     __shared__ double red_sh[2 * (NT / 64)];
     __shared__ T peak_sh[NT / 64];
     __shared__ int kpk_sh[NT / 64];
@@ -1448,22 +1453,22 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
                          const T *y, T shift, double vm, int cover) __attribute__((always_inline)) {
         int sexp = 0;
         if (vm <= 1e300) {  // false for NaN and inf
-            // first attempt: 4x the densest window of the original points;
-            // second: nc points in one window, which cannot overflow
+            // No cell receives more than cnt weights of size <= vm * wscale, so
+            // with cnt * vm * wscale * 2^sexp <= 2^headroom no cell can wrap.
+            // First attempt: cnt = 4x the densest window of the original
+            // points.  The draws' own densest window is counted on the way
+            // (draws per block of 32 anchors; a cell's taps come from at most
+            // 32 consecutive anchors, so from two adjacent blocks): if it
+            // exceeds cnt the item is redone at that exact count.
+            int cnt = 4 * max(cover, 1);
             for (int attempt = 0; attempt < 2; ++attempt) {
-                const int cnt = attempt == 0 ? 4 * max(cover, 1) : max(nc, 1);
                 const double bound = vm * (double)cnt * (double)A.wscale;
                 const int e = bound > 0 ? (int)ceil(log2(bound)) : -kMaxExp;
                 sexp = min(max(FixedOf<T>::headroom_bits - e, -kMaxExp), kMaxExp);
                 const T mul = (T)ldexp((double)A.wscale, sexp);
                 for (int i = tid; i < G; i += NT) gu[i] = U(0);
-                if (tid == 0) overflow_sh = 0;
+                for (int i = tid; i < NB; i += NT) hist_sh[i] = 0;
                 __syncthreads();
-                // With cnt >= 4 every addition is below 2^(bits-4) in size, so a
-                // cell can only wrap after a returned value reached 2^(bits-2) in
-                // size: adding 2^(bits-2) sets the top bit exactly for those.
-                constexpr U kQuarter = U(1) << (8 * sizeof(U) - 2);
-                U flag = 0;
                 for (int p0 = 0; p0 < nc; p0 += NT * PB) {
                     PointRec<T> pr[PB];
                     T vv[PB];
@@ -1493,19 +1498,28 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
                             tap_weights<T, NTAPS>(vv[k] * mul, pr[k].e, A.bi, A.e3, w);
                             const int base = pr[k].a - Q0;
 #pragma unroll
-                            for (int q = 0; q < NTAPS; ++q) {
-                                const I add = FixedOf<T>::round(w[q]);
-                                const U old = atomicAdd(&gu[(base + q) & (G - 1)], (U)add);
-                                flag |= old + kQuarter;
-                            }
+                            for (int q = 0; q < NTAPS; ++q)
+                                atomicAdd(&gu[(base + q) & (G - 1)], (U)FixedOf<T>::round(w[q]));
+                            atomicAdd(&hist_sh[pr[k].a >> 5], 1);
                         }
                     }
                 }
-                if (flag >> (8 * sizeof(U) - 1)) overflow_sh = 1;
                 __syncthreads();
-                const bool redo = overflow_sh != 0;
+                int dmax = 0;
+                for (int b = tid; b < NB; b += NT)
+                    dmax = max(dmax, hist_sh[b] + hist_sh[(b + 1) & (NB - 1)]);
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o));
+                if ((tid & 63) == 0) dmax_sh[tid >> 6] = dmax;
                 __syncthreads();
-                if (!redo) break;
+                dmax = dmax_sh[0];
+#pragma unroll
+                for (int w = 1; w < NT / 64; ++w) dmax = max(dmax, dmax_sh[w]);
+                // every thread has read dmax_sh before the next attempt or item
+                // writes it (the FFT and the emit below have barriers too)
+                __syncthreads();
+                if (dmax <= cnt) break;  // uniform
+                cnt = dmax;
             }
             const T inv = (T)ldexp(1.0, -sexp);
             if constexpr (MODE == 0) {
