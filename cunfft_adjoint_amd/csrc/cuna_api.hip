@@ -582,12 +582,12 @@ void Plan<T>::forward(const cplx<T> *fhat, cplx<T> *f, hipStream_t s) {
     }
 }
 
-// Spread tiles: 256 cells for the one-wave FP32 scatter, 512 for the FP64
-// workgroup scatter (the fastest measured at N = 1e6, G = 2^21); many small
-// tiles keep the chip busy and bound the work of the densest tile
+// Spread tiles of 256 cells for the one-wave scatter in both libraries (the
+// fastest measured at N = 1e6 clustered points, G = 2^21); many small tiles
+// keep the chip busy and bound the work of the densest tile
 template <typename T>
 static int default_tlog() {
-    return sizeof(T) == 4 ? 8 : 9;
+    return 8;
 }
 
 template <typename T>

@@ -410,28 +410,19 @@ static void launch_timed(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s, 
     CUNA_LAUNCH_CHECK();
 }
 
-// The scatter forms: FP32 accumulates per wave in 32-bit cells
-// (k_spread_wv, one wave per tile), FP64 per workgroup in 64-bit cells
-// (k_spread_fx); both fixed point, neither needs anything measured at plan
-// time to be safe.  CUNA_PLAN_GATHER selects the gather form.
+// The scatter form (default): one wave per tile, fixed-point cells per wave
+// (32-bit in FP32, 64-bit in FP64), nothing measured at plan time needed to be
+// safe.  CUNA_PLAN_GATHER selects the gather form.
 template <typename T, int NTAPS, int VM>
 static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s, hipEvent_t ev0,
                        hipEvent_t ev1) {
     if (!A.gather) {
-        if constexpr (sizeof(T) == 4) {
-            constexpr int WPT = 1;
-            const size_t lds = spread_wv_lds_bytes<T>(A.tlog, NTAPS, WPT);
-            require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
-            auto k = k_spread_wv<T, NTAPS, VM, WPT>;
-            set_lds(k, lds);
-            launch_timed(k, dim3(A.ntiles, nslabs), dim3(64 * WPT), lds, s, ev0, ev1, A);
-        } else {
-            const size_t lds = spread_fx_lds_bytes<T>(A.tlog, NTAPS);
-            require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
-            auto k = k_spread_fx<T, NTAPS, VM>;
-            set_lds(k, lds);
-            launch_timed(k, dim3(A.ntiles, nslabs), dim3(kSpreadFxThreads), lds, s, ev0, ev1, A);
-        }
+        constexpr int WPT = 1;
+        const size_t lds = spread_wv_lds_bytes<T>(A.tlog, NTAPS, WPT);
+        require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
+        auto k = k_spread_wv<T, NTAPS, VM, WPT>;
+        set_lds(k, lds);
+        launch_timed(k, dim3(A.ntiles, nslabs), dim3(64 * WPT), lds, s, ev0, ev1, A);
         return;
     }
     const size_t lds = spread_lds_bytes<T>(A.tlog, NTAPS, A.cap);

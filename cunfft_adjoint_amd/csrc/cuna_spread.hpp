@@ -10,9 +10,9 @@
 // boundary classes of its two neighbours and accumulates the taps that land
 // in its tile in LDS, then writes the tile once with coalesced stores.  No
 // global atomics, and every accumulation form is bitwise reproducible:
-//   k_spread_wv (FP32 default): one wave per tile, 32-bit integer atomics in
-//     fixed point (integer sums do not depend on their order);
-//   k_spread_fx (FP64 default): one workgroup per tile, 64-bit cells;
+//   k_spread_wv (default): one wave per tile, integer atomics in fixed point
+//     (32-bit cells in FP32, 64-bit in FP64; integer sums do not depend on
+//     their order), tiles taken densest first;
 //   k_spread (on request, CUNA_PLAN_GATHER): gather, each cell summed by
 //     one thread in a fixed order.
 // The fused bootstrap kernel (k_boot_fused) keeps a whole curve's grid in
@@ -899,186 +899,6 @@ __device__ __forceinline__ void tap_fixed(T v, T e, T bi, const T *e3, T c1,
     }
 }
 
-
-// threads per workgroup and point slots per thread of the fixed-point scatter
-constexpr int kSpreadFxThreads = 256, kSpreadFxSlots = 4;
-
-template <typename T, int NTAPS, int VM, class FX>
-__device__ __forceinline__ void spread_fx_body(const SpreadArgs<T> &A, T *red, int cover) {
-    constexpr int NT = kSpreadFxThreads, PPT = kSpreadFxSlots, CAP = NT * PPT;
-    constexpr int Q0 = (NTAPS - 1) / 2, HALO = NTAPS;
-    const int TT = 1 << A.tlog;
-    const int NC = TT + 2 * HALO;  // cells -HALO .. TT+HALO-1
-    using Acc = typename FX::U;
-    using SAcc = std::make_signed_t<Acc>;
-    Acc *acc = reinterpret_cast<Acc *>(cuna_smem);
-    // TT, for tiles of several rounds (after room for 64-bit cells)
-    T *facc = reinterpret_cast<T *>(reinterpret_cast<unsigned long long *>(cuna_smem) + NC);
-    // finite values whose products with cnt and wscale stay finite
-    constexpr T kBig = sizeof(T) == 4 ? T(3.0e38f) : T(1.0e300);
-    // 2^sexp also keeps wscale * 2^sexp finite
-    constexpr int kMaxExp = sizeof(T) == 4 ? 100 : 1000;
-
-    // consecutive workgroups go to different XCDs (hardware round-robin):
-    // the densest tiles, first in the work order, are spread over all of them
-    const int slab = blockIdx.y;
-    const int c = slab / A.nrep, rep = slab % A.nrep;
-    TileRanges R;
-    if (A.tile_desc)
-        R.from_desc(A, ldc_desc(A.tile_desc + (long)c * A.ntiles + blockIdx.x), NTAPS);
-    else
-        R.init(A, blockIdx.x, c, NTAPS);
-    const int total = R.total;
-    const bool multi = total > CAP;
-    const int tid = threadIdx.x;
-    // exp(-bi): e3[q] = exp(-(q - Q0)^2 bi)
-    const T c1 = A.cpar ? A.cpar[3 * c + 2] : A.e3[Q0 + 1];
-    const T bi = A.cpar ? A.cpar[3 * c] : A.bi;
-    const T wscale = A.cpar ? A.cpar[3 * c + 1] : A.wscale;
-
-    const int tlen = min(TT, A.G - R.tstart);  // cells of the tile inside the grid
-    for (int g = tid; g < NC; g += NT) acc[g] = Acc(0);
-    if (multi)
-        for (int g = tid; g < TT; g += NT) facc[g] = T(0);
-    // the cells a thread reads back were cleared by other threads: a tile
-    // with no points has no other barrier before the final read, and would
-    // otherwise see what an earlier workgroup left in this LDS
-    __syncthreads();
-    bool finite = true;
-    int sexp = 0;
-    for (int base = 0; base < total; base += CAP) {
-        const int cnt = min(CAP, total - base);
-        // Branch-free loads: a slot past the round's end reads the round's
-        // first point and is masked afterwards, so that all PPT slots'
-        // loads issue together (a branch per slot would wait out each one).
-        // Slot i of the round takes point (i P) mod cnt, P a prime not dividing
-        // cnt (a bijection): the lanes of one wave take points ~P apart in
-        // anchor order rather than neighbours, which often share a cell and
-        // then serialise their atomics.
-        const int P = cnt % 97 ? 97 : 101;
-        int am[PPT], pk[PPT];
-        T ev[PPT], vv[PPT];
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            const int i = k * NT + tid;
-            int s, ro;
-            R.get(base + (i < cnt ? (i * P) % cnt : 0), s, ro);
-            const PointRec<T> pr = A.rec[s];
-            am[k] = pr.a - ro;
-            ev[k] = pr.e;
-            pk[k] = A.perm[s];
-        }
-        T vmax = T(0);
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            T v, e = ev[k];
-            point_value<T, VM>(A, pk[k], c, rep, v, e);
-            const bool ok = base + k * NT + tid < base + cnt;
-            vv[k] = ok ? v : T(0);
-            ev[k] = e;
-            // a non-finite value or position (NaN x gives NaN e) makes the
-            // round's maximum infinite, and the tile's output NaN
-            T av = fabs(vv[k]);
-            if constexpr (NTAPS % 2 == 1) {
-                // compat filter: e lies up to R_f + 1 cells from the taps and
-                // taps off the grid are dropped, so the largest weight this
-                // tile keeps, at its tap o nearest e, can be far below |v|
-                // (accurate filters always have a tap within half a cell)
-                const int lo = max(-Q0, -am[k]), hi = min(NTAPS - 1 - Q0, tlen - 1 - am[k]);
-                const T o = fmin(fmax(rint(e), T(lo)), T(hi));
-                av = lo <= hi ? av * dexp(-(o - e) * (o - e) * bi) : T(0);
-            }
-            if (!(av <= kBig) || (ok && !(fabs(e) <= kBig))) av = T(INFINITY);
-            vmax = fmax(vmax, av);
-        }
-        // round maximum of |v| (compat: of each point's largest weight / wscale)
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
-        if ((tid & 63) == 0) red[tid >> 6] = vmax;
-        __syncthreads();
-        vmax = red[0];
-#pragma unroll
-        for (int w = 1; w < NT / 64; ++w) vmax = fmax(vmax, red[w]);
-        if (!(vmax <= kBig)) finite = false;
-        // 2^sexp: cnt * vmax * wscale * 2^sexp <= 2^headroom, and wscale * 2^sexp <= 2^kMaxExp
-        const T bound = (T)min(cnt, cover) * vmax * wscale;
-        int eb = 0;
-        if (bound > T(0) && finite) (void)frexp(bound, &eb);
-        else eb = -64;
-        int ew = 0;
-        (void)frexp(wscale, &ew);
-        sexp = min(FX::headroom - eb, kMaxExp - ew);
-        if constexpr (FX::single > 0) {
-            // and every weight on the tile's cells within 2^single
-            int e1 = -64;
-            const T b1 = vmax * wscale;
-            if (b1 > T(0) && finite) (void)frexp(b1, &e1);
-            sexp = min(sexp, FX::single - e1);
-        }
-        const T mul = ldexp(wscale, sexp);
-        if (finite) {
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                if (base + k * NT + tid < base + cnt) {
-                    Acc nq[NTAPS];
-                    // accurate filters (even NTAPS) share one table e3; compat
-                    // batches carry a filter per curve (cpar), so recurrences
-                    tap_fixed<T, NTAPS, FX, NTAPS % 2 == 0>(vv[k] * mul, ev[k], bi, A.e3, c1, nq);
-                    const int c0 = am[k] - Q0;  // first tap's cell
-                    if (R.tiny) {
-#pragma unroll
-                        for (int q = 0; q < NTAPS; ++q) {
-                            const int cell = c0 + q;
-                            if (cell >= -HALO && cell < TT + HALO) atomicAdd(&acc[HALO + cell], nq[q]);
-                        }
-                    } else {
-                        Acc *dst = acc + HALO + c0;
-#pragma unroll
-                        for (int q = 0; q < NTAPS; ++q) atomicAdd(dst + q, nq[q]);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        if (multi) {
-            // fold the round into facc; each thread clears the cells it read
-            // (a different thread-to-cell map for the clear would race with
-            // the reads of other waves), and the halo cells no thread reads
-            const T inv = ldexp(T(1), -sexp);
-            for (int g = tid; g < TT; g += NT) {
-                facc[g] += (T)(SAcc)acc[HALO + g] * inv;
-                acc[HALO + g] = Acc(0);
-            }
-            for (int h = tid; h < 2 * HALO; h += NT) acc[h < HALO ? h : TT + h] = Acc(0);
-            __syncthreads();
-        }
-    }
-    T *gout = A.grid + (long)slab * A.grid_stride + R.tstart;
-    const T inv = ldexp(T(1), -sexp);
-    for (int g = tid; g < tlen; g += NT) {
-        const T v = multi ? facc[g] : (T)(SAcc)acc[HALO + g] * inv;
-        gout[g] = finite ? v : T(NAN);
-    }
-}
-
-// Scatter kernel of the FP64 library: one workgroup per tile, 64-bit cells.
-template <typename T, int NTAPS, int VM>
-__global__ __launch_bounds__(kSpreadFxThreads) void k_spread_fx(SpreadArgs<T> A) {
-    __shared__ T red[kSpreadFxThreads / 64];
-    // no cell of the tile receives more than `cover` weights in a round
-    const int c = blockIdx.y / A.nrep;
-    const long e = (long)c * A.ntiles + blockIdx.x;
-    const int cover = A.tile_desc    ? ldc(&A.tile_desc[e].cover)
-                      : A.tile_cover ? ldc(A.tile_cover + e)
-                                     : kSpreadFxThreads * kSpreadFxSlots;
-    spread_fx_body<T, NTAPS, VM, Fixed64<T>>(A, red, cover);
-}
-
-template <typename T>
-__host__ __device__ inline size_t spread_fx_lds_bytes(int tlog, int ntaps) {
-    const size_t tt = (size_t)1 << tlog;
-    return sizeof(unsigned long long) * (tt + 2 * (size_t)ntaps) + sizeof(T) * tt;
-}
 
 // ------------------------------------------- per-wave fixed-point scatter --
 // Each wave of the tile's workgroup accumulates its own share of the points
