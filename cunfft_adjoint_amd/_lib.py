@@ -111,6 +111,7 @@ def _protos(prec: str):
         "cuna_nfft_plan_create_batch": (c_int, [P, P, P, c_int, c_int, P, P]),
         "cuna_nfft_plan_create_batch_times": (c_int, [P, P, P, c_int, c_double, c_int, P, P]),
         "cuna_nfft_adjoint": (c_int, [P, P, P, P]),
+        "cuna_nfft_adjoint_many": (c_int, [P, P, c_int, P, P]),
         "cuna_nfft_bootstrap": (c_int, [P, P, c_int, c_uint, P, P, P]),
         "cuna_nfft_bootstrap_ls": (c_int, [P, P, c_int, c_uint, P, P, P, P]),
         "cuna_nfft_forward": (c_int, [P, P, P, P]),

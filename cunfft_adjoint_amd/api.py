@@ -348,6 +348,16 @@ class NFFTPlan:
         check(self.L, self.L.cuna_nfft_adjoint(self.h, _ptr(y), _ptr(out), _stream(stream)))
         return out
 
+    def adjoint_many(self, y, out=None, stream=None):
+        """y: (nrhs, n) values; returns (ncurves * nrhs, M), row c * nrhs + r
+        the spectrum of right-hand side r for curve c."""
+        nrhs = int(y.shape[0])
+        if out is None:
+            out = self._out(self.ncurves * nrhs)
+        check(self.L, self.L.cuna_nfft_adjoint_many(self.h, _ptr(y), nrhs, _ptr(out),
+                                                    _stream(stream)))
+        return out
+
     def bootstrap(self, y, nbs: int, seed: int = 0, window: bool = False, stream=None,
                   out_y=None, out_w=None):
         if out_y is None:

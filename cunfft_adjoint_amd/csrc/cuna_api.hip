@@ -325,6 +325,30 @@ void Plan<T>::adjoint_into(const T *y, cplx<T> *out, T *grid_ptr, hipStream_t s)
     if (prof[3]) CUNA_HIP(hipEventRecord(prof[3], s));
 }
 
+// Several right-hand sides of one plan: slab c * nrhs + r of the spread takes
+// curve c's points with values y[r * n + p] (the spread's per-replicate value
+// stride), and the FFT transforms all slabs in one batch, so row c * nrhs + r
+// of out equals cuna_nfft_adjoint of row r of y, curve c.
+template <typename T>
+void Plan<T>::adjoint_many(const T *y, int nrhs, cplx<T> *out, hipStream_t s) {
+    require(spec.mode == CUNA_MODE_ACCURATE, "several right-hand sides need an accurate-mode plan");
+    require(y != nullptr && out != nullptr, "NULL pointer");
+    require(nrhs >= 1, "nrhs must be positive");
+    const int nc = ncurves();
+    require((long)nc * nrhs <= 65535, "ncurves * nrhs must be at most 65535",
+            CUNA_ERROR_UNSUPPORTED);
+    grid.ensure(sizeof(T) * (size_t)G * nc * nrhs);
+    SpreadArgs<T> A = spread_args(pts);
+    A.y = y;
+    A.y_slab_stride = n();
+    A.nrep = nrhs;
+    A.grid = grid.as<T>();
+    A.grid_stride = G;
+    launch_spread<T>(A, ntaps, VAL_PLAIN, nc * nrhs, s, prof[0], prof[1]);
+    run_fft(grid.as<T>(), out, nc * nrhs, s, prof[2]);
+    if (prof[3]) CUNA_HIP(hipEventRecord(prof[3], s));
+}
+
 template <typename T>
 void Plan<T>::bootstrap_compat(const T *y, cplx<T> *out, int nbs, uint32_t seed, hipStream_t s) {
     require(spec.mode == CUNA_MODE_COMPAT && ncurves() == 1, "compat bootstrap needs a compat plan");
@@ -1421,6 +1445,14 @@ CUNA_API int cuna_nfft_adjoint(cuna_nfft_plan plan, const dTyp *y, Complex *out,
     return guarded([&] {
         require(plan && plan->p, "plan is NULL");
         plan->p->adjoint(y, reinterpret_cast<cplx<R> *>(out), stream);
+    });
+}
+
+CUNA_API int cuna_nfft_adjoint_many(cuna_nfft_plan plan, const dTyp *y, int nrhs, Complex *out,
+                                    hipStream_t stream) {
+    return guarded([&] {
+        require(plan && plan->p, "plan is NULL");
+        plan->p->adjoint_many(y, nrhs, reinterpret_cast<cplx<R> *>(out), stream);
     });
 }
 

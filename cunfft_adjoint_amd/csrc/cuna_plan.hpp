@@ -140,6 +140,9 @@ struct Plan {
     // the same with the real grid (ncurves x G) in caller-provided memory;
     // allocates nothing
     void adjoint_into(const T *y, cplx<T> *out, T *grid_ptr, hipStream_t s) const;
+    // nrhs value vectors (rows of n) -> out (ncurves x nrhs x M), one spread
+    // launch and one batched FFT
+    void adjoint_many(const T *y, int nrhs, cplx<T> *out, hipStream_t s);
     // compat bootstrap of src/cuna_kernel.cu:236-289 (single curve)
     void bootstrap_compat(const T *y, cplx<T> *out, int nbs, uint32_t seed, hipStream_t s);
     // accurate bootstrap (resampled points)

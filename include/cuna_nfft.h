@@ -88,6 +88,15 @@ int cuna_nfft_plan_create_batch_times(cuna_nfft_plan *plan, const double *t, con
 /* out: ncurves x M complex (curve-major).  y: n values or NULL (unit weights). */
 int cuna_nfft_adjoint(cuna_nfft_plan plan, const dTyp *y, Complex *out, hipStream_t stream);
 
+/* accurate mode: nrhs value vectors of one plan in one spread launch and one
+ * batched FFT.  y: nrhs x n (row r holds the values of right-hand side r, in
+ * the plan's point order); out: ncurves x nrhs x M complex, row c*nrhs + r
+ * equal to cuna_nfft_adjoint of row r for curve c.  ncurves * nrhs <= 65535.
+ * No counterpart in the reference (its batching is over bootstrap replicates,
+ * src/cuna_kernel.cu:236-289). */
+int cuna_nfft_adjoint_many(cuna_nfft_plan plan, const dTyp *y, int nrhs, Complex *out,
+                           hipStream_t stream);
+
 /* accurate mode, per plan: nbs bootstrap replicates.  Replicate s of curve c
  * draws d = offsets[c] + xorshift(wang(seed + s*n_c + p)) % n_c for
  * p = 0 .. n_c-1 (the reference's generator, src/gridding.cu:67-101).
