@@ -186,6 +186,16 @@ __device__ __forceinline__ void apply_twiddles(cplx<T> *v, const TW &tw, int k) 
     }
 }
 
+// Workgroup barrier that orders LDS only: each wave waits for its own LDS
+// operations, not for its outstanding global loads.  The FFT's passes
+// exchange data through LDS alone, so global loads issued earlier (the
+// epilogue's twiddles) keep flying across the passes.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ----------------------------------------------------------- LDS Stockham --
 // W interleaved sequences of length L = 2^LG: element e of sequence w lives at
 // lds[lds_sw(e*W + w)].  Pass 0 reads its inputs through `ld(e, w)`; every pass writes
@@ -241,7 +251,7 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, TW tw, const Loader &ld,
         if constexpr (P == 0) hook();
         // pass 0 reads through the loader; when that reads this same LDS
         // buffer (INPLACE) all reads must finish before the first write
-        if constexpr (P > 0 || INPLACE) __syncthreads();
+        if constexpr (P > 0 || INPLACE) lds_barrier();
 #pragma unroll
         for (int q = 0; q < BPT; ++q) {
             const int g = tid + q * NT;
@@ -256,7 +266,7 @@ __device__ __forceinline__ void fft_pass(cplx<T> *lds, TW tw, const Loader &ld,
                 for (int r = 0; r < R; ++r) lds[lds_sw((base + (r << LOGNS)) * W + w)] = v[q][r];
             }
         }
-        __syncthreads();
+        lds_barrier();
         fft_pass<T, S, LG, W, NT, P + 1, LOGNS + LR, Loader, INPLACE, Hook, TW, MR>(lds, tw, ld,
                                                                                    hook);
     }
