@@ -139,6 +139,32 @@ def copy_bandwidth(torch):
     return 2 * 4 * n / s
 
 
+def multi_rhs(plan, yd, out, M, N, info, nrhs, stream, ev, outer, K):
+    """nrhs value vectors of the same plan per call (cuna_nfft_adjoint_many)."""
+    import torch
+
+    sp = stream.cuda_stream
+    ym = torch.stack([yd * (1.0 + 0.125 * r) for r in range(nrhs)]).contiguous()
+    outm = torch.empty((nrhs, M), dtype=torch.complex64, device="cuda")
+    plan.adjoint_many(ym, out=outm, stream=stream)
+    stream.synchronize()
+    Km = max(4, K // nrhs)
+    ev.record(outer[0], sp)
+    for _ in range(Km):
+        plan.adjoint_many(ym, out=outm, stream=stream)
+    ev.record(outer[1], sp)
+    stream.synchronize()
+    ms = ev.ms(outer[0], outer[1]) / Km
+    diff = float(torch.linalg.vector_norm(outm[0] - out[0]) / torch.linalg.vector_norm(out[0]))
+    return {
+        "what": f"{nrhs} value vectors of the same plan per call (cuna_nfft_adjoint_many): one "
+                "spread launch and one batched FFT; not the headline, which is one vector per call",
+        "nrhs": nrhs, "ms_per_call": ms, "adj_nfft_per_s": nrhs * 1e3 / ms,
+        "first_row_rel_diff_vs_single": diff,
+        "frac_of_hbm_peak": nrhs * (N * (4 + 4) + 2 * info.G * 4 + M * 8) / (ms * 1e-3) / HBM_PEAK,
+    }
+
+
 def bench_c2(args, torch, cu):
     from tests._data import clustered_points, rel_l2
     import oracle
@@ -228,20 +254,10 @@ def bench_c2(args, torch, cu):
     graph.close()
     # several right-hand sides of the same plan per call (cuna_nfft_adjoint_many):
     # one spread launch and one batched FFT for all of them
-    nrhs = 8
-    ym = torch.stack([yd * (1.0 + 0.125 * r) for r in range(nrhs)]).contiguous()
-    outm = torch.empty((nrhs, M), dtype=torch.complex64, device="cuda")
-    plan.adjoint_many(ym, out=outm, stream=stream)
-    stream.synchronize()
-    Km = max(4, K // nrhs)
-    ev.record(outer[0], sp)
-    for _ in range(Km):
-        plan.adjoint_many(ym, out=outm, stream=stream)
-    ev.record(outer[1], sp)
-    stream.synchronize()
-    many_ms = ev.ms(outer[0], outer[1]) / Km
-    many_diff = float(torch.linalg.vector_norm(outm[0] - out[0]) / torch.linalg.vector_norm(out[0]))
-    del ym, outm
+    multi = None
+    nrhs = 8 if not args.no_multi_rhs else 0
+    if nrhs:
+        multi = multi_rhs(plan, yd, out, M, N, info, nrhs, stream, ev, outer, K)
     # eager launches from Python, for reference (host-bound at this size)
     ev.record(outer[0], sp)
     for _ in range(K):
@@ -279,15 +295,7 @@ def bench_c2(args, torch, cu):
         "plan_ms": plan_ms,
         "rel_err_vs_ndft": err,
         "stages_ms": {k: v[0] for k, v in kernels.items()},
-        "multi_rhs": {
-            "what": f"{nrhs} value vectors of the same plan per call (cuna_nfft_adjoint_many): one "
-                    "spread launch and one batched FFT; not the headline, which is one vector "
-                    "per call",
-            "nrhs": nrhs, "ms_per_call": many_ms, "adj_nfft_per_s": nrhs * 1e3 / many_ms,
-            "first_row_rel_diff_vs_single": many_diff,
-            "frac_of_hbm_peak": nrhs * (N * (4 + 4) + 2 * info.G * 4 + M * 8)
-                                / (many_ms * 1e-3) / HBM_PEAK,
-        },
+        "multi_rhs": multi,
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
@@ -717,6 +725,9 @@ def main(argv=None):
                     help="first run one device copy of exactly 256 MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--boot-steps", type=int, default=4)
+    ap.add_argument("--no-multi-rhs", action="store_true",
+                    help="skip the several-value-vectors measurement (keeps a profile of the "
+                         "headline kernels to one-vector calls)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: it measures the single-GPU kernels. Over several "
