@@ -131,7 +131,7 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
         }
         total_chunks += k;
     }
-    const long hist_len = (long)std::max(K1, Kc) * std::max(total_chunks, 1);
+    const long hist_len = (long)std::max(K1, 256) * std::max(total_chunks, 1);
     coff.alloc(sizeof(int) * offsets.size());
     rec.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1));
     perm.alloc(sizeof(int) * std::max<long>(n, 1));
@@ -170,28 +170,30 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
     W.total_chunks = total_chunks;
     W.hist = hist.as<int>();
     W.bsum = bsum.as<int>();
-    // phase reduction / compat decisions, then an LSD sort: by the cell within
-    // the tile, then (stably) by tile and edge class -> sorted by anchor
+    // phase reduction / compat decisions, then an LSD sort by anchor: the cell
+    // within the tile, then the tile index in digits of at most 8 bits; the
+    // passes alternate between rec/perm and recA/permA, ending in rec/perm
     A.n = n;
     A.G = G;
     A.tlog = tlog;
     A.rec = rec_orig.as<PointRec<T>>();
     A.key = keys.as<int>();
     launch_point_setup<T>(A, s);
-    launch_sort_pass<T>(W, K1, keys.as<int>(), rec_orig.as<PointRec<T>>(), nullptr,
-                        recA.as<PointRec<T>>(), permA.as<int>(), s);
-    launch_tile_keys<T>(recA.as<PointRec<T>>(), n, tlog, olo, olo + ntaps - 1, G, keys.as<int>(),
-                        s);
-    launch_sort_pass<T>(W, Kc, keys.as<int>(), recA.as<PointRec<T>>(), permA.as<int>(),
-                        rec.as<PointRec<T>>(), perm.as<int>(), s);
-    if (chunks.empty()) {
-        long nbk = (long)ncurves * (Kc + 1);
-        hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nbk, 256)), dim3(256), 0, s,
-                           bucket.as<int>(), nbk, 0);
-        CUNA_LAUNCH_CHECK();
-    } else {
-        launch_buckets(W, Kc, bucket.as<int>(), s);
+    const int tbits = ilog2(ntiles);
+    const int npass = 1 + (tbits + 7) / 8;
+    PointRec<T> *const rbuf[2] = {rec.as<PointRec<T>>(), recA.as<PointRec<T>>()};
+    int *const pbuf[2] = {perm.as<int>(), permA.as<int>()};
+    auto dst = [&](int p) { return (npass - 1 - p) & 1; };
+    launch_sort_pass<T>(W, K1, keys.as<int>(), rec_orig.as<PointRec<T>>(), nullptr, rbuf[dst(0)],
+                        pbuf[dst(0)], s);
+    for (int p = 1; p < npass; ++p) {
+        const int shift = 8 * (p - 1), bits = std::min(8, tbits - shift);
+        launch_digit_keys<T>(rbuf[dst(p - 1)], n, tlog + shift, (1 << bits) - 1, keys.as<int>(), s);
+        launch_sort_pass<T>(W, 1 << bits, keys.as<int>(), rbuf[dst(p - 1)], pbuf[dst(p - 1)],
+                            rbuf[dst(p)], pbuf[dst(p)], s);
     }
+    launch_buckets<T>(coff.as<int>(), ncurves, Kc, rec.as<PointRec<T>>(), tlog, olo,
+                      olo + ntaps - 1, G, bucket.as<int>(), s);
     // the spread's per-tile cover (bounds the sums any cell receives)
     {
         SpreadArgs<T> C{};

@@ -338,11 +338,11 @@ void launch_point_setup(const SetupArgs<T> &A, hipStream_t s) {
 }
 
 template <typename T>
-void launch_tile_keys(const PointRec<T> *rec, long n, int tlog, int olo, int ohi, int G, int *key,
-                      hipStream_t s) {
+void launch_digit_keys(const PointRec<T> *rec, long n, int shift, int mask, int *key,
+                       hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_tile_keys<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, rec, n, tlog, olo,
-                       ohi, G, key);
+    hipLaunchKernelGGL(k_digit_keys<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, rec, n, shift,
+                       mask, key);
     CUNA_LAUNCH_CHECK();
 }
 
@@ -389,11 +389,13 @@ void launch_sort_pass(const SortWork &W, int K, const int *keys, const PointRec<
     CUNA_LAUNCH_CHECK();
 }
 
-void launch_buckets(const SortWork &W, int K, int *bucket, hipStream_t s) {
-    const SortArgs S = sort_args(W, K, nullptr);
-    long nbk = (long)W.ncurves * (K + 1);
-    hipLaunchKernelGGL(k_buckets, dim3(ceil_div(nbk, 256)), dim3(256), 0, s, S, W.coff, W.ncurves,
-                       bucket);
+template <typename T>
+void launch_buckets(const int *coff, int ncurves, int Kc, const PointRec<T> *rec, int tlog,
+                    int olo, int ohi, int G, int *bucket, hipStream_t s) {
+    const long nbk = (long)ncurves * (Kc + 1);
+    if (nbk <= 0) return;
+    hipLaunchKernelGGL(k_buckets<T>, dim3(ceil_div(nbk, 256)), dim3(256), 0, s, coff, ncurves, Kc,
+                       rec, tlog, olo, ohi, G, bucket);
     CUNA_LAUNCH_CHECK();
 }
 
@@ -489,13 +491,10 @@ void launch_tile_cover(const SpreadArgs<T> &A, int ncurves, int *cover, hipStrea
 void launch_tile_order(const int *bucket, int Kc, int ntiles, int ncurves, bool periodic,
                        const int *cover, TileDesc *desc, hipStream_t s) {
     if (ncurves <= 0 || ntiles <= 0) return;
-    int P2 = 1;
-    while (P2 < ntiles) P2 <<= 1;
-    require(P2 <= 8192, "too many spread tiles for the work order", CUNA_ERROR_UNSUPPORTED);
-    const size_t lds = 2 * sizeof(int) * (size_t)P2;
-    set_lds(k_tile_order, lds);
-    hipLaunchKernelGGL(k_tile_order, dim3(ncurves), dim3(kOrderThreads), lds, s, bucket, Kc,
-                       ntiles, P2, periodic ? 1 : 0, cover, desc);
+    require(ntiles <= kOrderMaxTiles, "too many spread tiles for the work order",
+            CUNA_ERROR_UNSUPPORTED);
+    hipLaunchKernelGGL(k_tile_order, dim3(ncurves), dim3(kOrderThreads), 0, s, bucket, Kc, ntiles,
+                       periodic ? 1 : 0, cover, desc);
     CUNA_LAUNCH_CHECK();
 }
 
@@ -603,8 +602,9 @@ template void fft_complex<dTyp>(const FftShape &, cplx<dTyp> *, cplx<dTyp> *, in
 template void launch_point_setup<dTyp>(const SetupArgs<dTyp> &, hipStream_t);
 template void launch_sort_pass<dTyp>(const SortWork &, int, const int *, const PointRec<dTyp> *,
                                      const int *, PointRec<dTyp> *, int *, hipStream_t);
-template void launch_tile_keys<dTyp>(const PointRec<dTyp> *, long, int, int, int, int, int *,
-                                     hipStream_t);
+template void launch_digit_keys<dTyp>(const PointRec<dTyp> *, long, int, int, int *, hipStream_t);
+template void launch_buckets<dTyp>(const int *, int, int, const PointRec<dTyp> *, int, int, int, int,
+                                   int *, hipStream_t);
 template void launch_spread<dTyp>(const SpreadArgs<dTyp> &, int, int, int, hipStream_t, hipEvent_t,
                                   hipEvent_t);
 template void launch_gather<dTyp>(const GatherArgs<dTyp> &, int, int, hipStream_t);

@@ -62,11 +62,15 @@ void launch_point_setup(const SetupArgs<T> &A, hipStream_t s);
 template <typename T>
 void launch_sort_pass(const SortWork &W, int K, const int *keys, const PointRec<T> *rec_in,
                       const int *perm_in, PointRec<T> *rec_out, int *perm_out, hipStream_t s);
+// key = (anchor >> shift) & mask, the digits of the tile index
 template <typename T>
-void launch_tile_keys(const PointRec<T> *rec, long n, int tlog, int olo, int ohi, int G, int *key,
-                      hipStream_t s);
-// bucket offsets of the last sort pass (over K keys)
-void launch_buckets(const SortWork &W, int K, int *bucket, hipStream_t s);
+void launch_digit_keys(const PointRec<T> *rec, long n, int shift, int mask, int *key,
+                       hipStream_t s);
+// first sorted index of every (curve, 3 * tile + edge class) key, from points
+// sorted by anchor
+template <typename T>
+void launch_buckets(const int *coff, int ncurves, int Kc, const PointRec<T> *rec, int tlog,
+                    int olo, int ohi, int G, int *bucket, hipStream_t s);
 
 // ev0/ev1 (optional) receive the spread kernel's own start and stop times
 template <typename T>

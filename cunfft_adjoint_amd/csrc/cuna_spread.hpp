@@ -167,13 +167,12 @@ __device__ __forceinline__ int tile_key(int a, int tlog, int olo, int ohi, int G
     return 3 * tile + cls;
 }
 
-// second sort digit: 3*tile + class (class 0: taps reach the previous tile,
-// 2: the next one, 1: interior)
+// sort digits after the first: bits [shift, shift + log2(mask + 1)) of the
+// tile index a >> tlog
 template <typename T>
-__global__ void k_tile_keys(const PointRec<T> *rec, long n, int tlog, int olo, int ohi, int G,
-                            int *key) {
+__global__ void k_digit_keys(const PointRec<T> *rec, long n, int shift, int mask, int *key) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) key[s] = tile_key(rec[s].a, tlog, olo, ohi, G);
+    if (s < n) key[s] = (rec[s].a >> shift) & mask;
 }
 
 // A stable counting sort pass over K keys.  Points are cut into chunks that
@@ -252,18 +251,25 @@ __global__ void k_scatter(SortArgs S, const PointRec<T> *rec_in, const int *perm
     }
 }
 
-// bucket[c*(K+1) + k] = first sorted index of key k of curve c
-static __global__ void k_buckets(SortArgs S, const int *coff, int ncurves, int *bucket) {
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long total = (long)ncurves * (S.K + 1);
-    if (idx >= total) return;
-    int c = (int)(idx / (S.K + 1)), k = (int)(idx % (S.K + 1));
-    int v;
-    if (k == S.K || S.nch[c] == 0)
-        v = k == S.K ? coff[c + 1] : coff[c];
-    else
-        v = S.hist[(long)S.K * S.chunk_base[c] + (long)k * S.nch[c]];
-    bucket[idx] = v;
+// bucket[c*(Kc+1) + k] = first sorted index of the curve's points with
+// tile_key >= k (k = Kc: the curve's end).  The points are sorted by anchor,
+// and tile_key is non-decreasing in the anchor (the class is 0 for a prefix
+// of the tile's cells and 2 for a suffix), so a binary search finds it.
+template <typename T>
+__global__ void k_buckets(const int *coff, int ncurves, int Kc, const PointRec<T> *rec, int tlog,
+                          int olo, int ohi, int G, int *bucket) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ncurves * (Kc + 1)) return;
+    const int c = (int)(idx / (Kc + 1)), k = (int)(idx % (Kc + 1));
+    int lo = k < Kc ? coff[c] : coff[c + 1], hi = coff[c + 1];
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (tile_key(rec[mid].a, tlog, olo, ohi, G) < k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    bucket[idx] = lo;
 }
 
 // ------------------------------------------------ exclusive int scan --
@@ -748,45 +754,81 @@ __global__ __launch_bounds__(256) void k_tile_cover(SpreadArgs<T> A, int *cover)
 }
 
 // Work order of the scatter kernels, one workgroup per curve: the curve's
-// tiles sorted by decreasing own point count (bitonic sort in LDS of P2 >=
-// ntiles keys, P2 a power of two <= 8192), then one TileDesc per position.
-constexpr int kOrderThreads = 1024;
+// tiles by decreasing own point count, as a stable counting sort over
+// kOrderBins bins of the count (exact below 128 points, in steps of 8 up to
+// 1143, one bin above), ties in tile order; then one TileDesc per position.
+// The order only schedules the tiles: every tile's cells come out the same
+// in any order.  Each of the 16 waves ranks a contiguous segment of tiles.
+constexpr int kOrderThreads = 1024, kOrderWaves = kOrderThreads / 64, kOrderBins = 256;
+constexpr int kOrderMaxTiles = 8192;
+static_assert(kOrderThreads == kScanNT, "the work order scans with one value per thread");
+
+__device__ __forceinline__ int order_bin(int npts) {
+    const int r = npts < 128 ? npts : 128 + min(127, (npts - 128) >> 3);
+    return kOrderBins - 1 - r;  // bin 0: the densest tiles
+}
+
 static __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int *bucket, int Kc,
-                                                                     int ntiles, int P2,
-                                                                     int periodic,
+                                                                     int ntiles, int periodic,
                                                                      const int *cover,
                                                                      TileDesc *desc) {
-    int *key = reinterpret_cast<int *>(cuna_smem);
-    int *val = key + P2;
+    __shared__ int cnt[kOrderBins * kOrderWaves];  // [bin][wave]: counts, then starts
+    __shared__ int scan_sh[kOrderThreads];
+    __shared__ unsigned char bin_sh[kOrderMaxTiles];
+    __shared__ short order[kOrderMaxTiles];
     const int c = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int *bk = bucket + (long)c * (Kc + 1);
-    for (int i = threadIdx.x; i < P2; i += kOrderThreads) {
-        key[i] = i < ntiles ? bk[3 * i + 3] - bk[3 * i] : -1;
-        val[i] = i;
+    for (int i = tid; i < kOrderBins * kOrderWaves; i += kOrderThreads) cnt[i] = 0;
+    for (int t = tid; t < ntiles; t += kOrderThreads)
+        bin_sh[t] = (unsigned char)order_bin(bk[3 * t + 3] - bk[3 * t]);
+    __syncthreads();
+    const int seg = (ntiles + kOrderWaves - 1) / kOrderWaves;
+    const int s0 = min(ntiles, wave * seg), s1 = min(ntiles, s0 + seg);
+    for (int t = s0 + lane; t < s1; t += 64) atomicAdd(&cnt[bin_sh[t] * kOrderWaves + wave], 1);
+    __syncthreads();
+    // exclusive scan in (bin, wave) order: a wave's tiles of one bin follow
+    // the earlier waves' tiles of that bin
+    constexpr int IPT = kOrderBins * kOrderWaves / kOrderThreads;
+    int v[IPT], sum = 0;
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+        v[q] = cnt[IPT * tid + q];
+        sum += v[q];
+    }
+    int total;
+    int ex = block_exclusive_scan(sum, scan_sh, total);
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+        cnt[IPT * tid + q] = ex;
+        ex += v[q];
     }
     __syncthreads();
-    // "before": more points, or as many and a smaller tile index
-    auto before = [&](int a, int b) { return key[a] > key[b] || (key[a] == key[b] && val[a] < val[b]); };
-    for (int k = 2; k <= P2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < P2; i += kOrderThreads) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const bool up = (i & k) == 0;
-                    if (up ? before(l, i) : before(i, l)) {
-                        const int tk = key[i], tv = val[i];
-                        key[i] = key[l];
-                        val[i] = val[l];
-                        key[l] = tk;
-                        val[l] = tv;
-                    }
-                }
-            }
-            __syncthreads();
+    // rank within the wave's 64 tiles of a step: lanes with the same bin
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int t0 = s0; t0 < s1; t0 += 64) {
+        const int t = t0 + lane;
+        const bool act = t < s1;
+        const int b = act ? bin_sh[t] : 0;
+        unsigned long long peers = __ballot(act);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const unsigned long long m = __ballot((b >> bit) & 1);
+            peers &= ((b >> bit) & 1) ? m : ~m;
         }
+        const int rank = __popcll(peers & lt);
+        int pos = 0;
+        if (act) pos = cnt[b * kOrderWaves + wave] + rank;
+        __builtin_amdgcn_wave_barrier();
+        if (act) {
+            order[pos] = (short)t;
+            if (rank == __popcll(peers) - 1) cnt[b * kOrderWaves + wave] = pos + 1;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
+    __syncthreads();
     for (int i = threadIdx.x; i < ntiles; i += kOrderThreads) {
-        const int t = val[i];
+        const int t = order[i];
         TileDesc d;
         d.tile = t;
         d.b1 = bk[3 * t];
