@@ -114,8 +114,10 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
     Kc = 3 * ntiles;
     const int K1 = 1 << tlog;
 
-    // chunk table: chunks of at most 2048 points, never crossing a curve
-    constexpr int CH = 2048;
+    // chunk table: chunks of at most CH points, never crossing a curve.  One
+    // wave ranks a chunk against per-chunk counts of every key: 512 points
+    // for the 256-key passes, up to 2048 for larger tiles (first digit)
+    const int CH = std::max(512, std::min(2048, 2 * K1));
     std::vector<int4> chunks;
     std::vector<int> base(ncurves), nch(ncurves);
     int total_chunks = 0;
@@ -132,44 +134,86 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
         total_chunks += k;
     }
     const long hist_len = (long)std::max(K1, 256) * std::max(total_chunks, 1);
-    coff.alloc(sizeof(int) * offsets.size());
-    rec.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1));
-    perm.alloc(sizeof(int) * std::max<long>(n, 1));
-    rec_orig.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1));
-    bucket.alloc(sizeof(int) * (size_t)ncurves * (Kc + 1));
+    // work-queue order of the curves for the fused batched kernels
+    std::vector<int> ord(ncurves);
+    for (int c = 0; c < ncurves; ++c) ord[c] = c;
+    std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
+        return offsets[u + 1] - offsets[u] > offsets[v + 1] - offsets[v];
+    });
+    // device memory in two allocations, the plan's buffers and the sort's
+    // stream-ordered temporaries, each filled by one upload: allocation and
+    // copy calls are most of a plan build's host time
+    struct Carve {
+        size_t total = 0;
+        size_t add(size_t b) {
+            const size_t o = total;
+            total += (b + 255) & ~size_t(255);
+            return o;
+        }
+    };
+    const long n1 = std::max<long>(n, 1);
+    const size_t ncov = (size_t)std::max(ncurves, 1) * std::max(ntiles, 1);
+    Carve pc;
+    const size_t o_up = pc.add(sizeof(int) * (2 * (size_t)ncurves + 1));  // coff, order
+    const size_t o_rec = pc.add(sizeof(PointRec<T>) * n1);
+    const size_t o_perm = pc.add(sizeof(int) * n1);
+    const size_t o_orig = pc.add(sizeof(PointRec<T>) * n1);
+    const size_t o_bk = pc.add(sizeof(int) * (size_t)ncurves * (Kc + 1));
+    const size_t o_eps = keep_eps ? pc.add(sizeof(T) * n1) : 0;
+    const size_t o_cov = pc.add(sizeof(int) * ncov);
+    const size_t o_desc = pc.add(sizeof(TileDesc) * ncov);
+    arena.alloc(pc.total);
+    char *pb = arena.as<char>();
+    coff.view(pb + o_up, sizeof(int) * (ncurves + 1));
+    order.view(pb + o_up + sizeof(int) * (ncurves + 1), sizeof(int) * ncurves);
+    rec.view(pb + o_rec, sizeof(PointRec<T>) * n1);
+    perm.view(pb + o_perm, sizeof(int) * n1);
+    rec_orig.view(pb + o_orig, sizeof(PointRec<T>) * n1);
+    bucket.view(pb + o_bk, sizeof(int) * (size_t)ncurves * (Kc + 1));
     if (keep_eps) {
-        eps_orig.alloc(sizeof(T) * std::max<long>(n, 1));
+        eps_orig.view(pb + o_eps, sizeof(T) * n1);
         A.eps_orig = eps_orig.as<T>();
+    } else {
+        eps_orig.release();
     }
-    CUNA_HIP(hipMemcpyAsync(coff.ptr, coff_h.data(), sizeof(int) * coff_h.size(),
+    tile_cover.view(pb + o_cov, sizeof(int) * ncov);
+    tile_desc.view(pb + o_desc, sizeof(TileDesc) * ncov);
+    plan_h.assign(offsets.begin(), offsets.end());
+    plan_h.insert(plan_h.end(), ord.begin(), ord.end());
+    CUNA_HIP(hipMemcpyAsync(pb + o_up, plan_h.data(), sizeof(int) * plan_h.size(),
                             hipMemcpyHostToDevice, s));
-    // temporaries, stream-ordered
-    DevBuf d_chunks, d_base, d_nch, keys, recA, permA, hist, bsum;
-    d_chunks.alloc(sizeof(int4) * std::max<size_t>(chunks.size(), 1), s);
-    d_base.alloc(sizeof(int) * ncurves, s);
-    d_nch.alloc(sizeof(int) * ncurves, s);
-    keys.alloc(sizeof(int) * std::max<long>(n, 1), s);
-    recA.alloc(sizeof(PointRec<T>) * std::max<long>(n, 1), s);
-    permA.alloc(sizeof(int) * std::max<long>(n, 1), s);
-    hist.alloc(sizeof(int) * hist_len, s);
-    bsum.alloc(sizeof(int) * ceil_div(hist_len, kScanTile), s);
-    if (!chunks.empty())
-        CUNA_HIP(hipMemcpyAsync(d_chunks.ptr, chunks.data(), sizeof(int4) * chunks.size(),
-                                hipMemcpyHostToDevice, s));
-    CUNA_HIP(hipMemcpyAsync(d_base.ptr, base.data(), sizeof(int) * ncurves, hipMemcpyHostToDevice,
+    Carve tc;
+    const size_t o_chunks = tc.add(sizeof(int4) * std::max<size_t>(chunks.size(), 1));
+    const size_t o_base = tc.add(sizeof(int) * ncurves);
+    const size_t o_nch = tc.add(sizeof(int) * ncurves);
+    const size_t o_keys = tc.add(sizeof(int) * n1);
+    const size_t o_recA = tc.add(sizeof(PointRec<T>) * n1);
+    const size_t o_permA = tc.add(sizeof(int) * n1);
+    const size_t o_hist = tc.add(sizeof(int) * hist_len);
+    const size_t o_bsum = tc.add(sizeof(int) * ceil_div(hist_len, kScanTile));
+    DevBuf tmp;
+    tmp.alloc(tc.total, s);
+    char *tb = tmp.as<char>();
+    // chunk table, chunk bases and counts, laid out as in the temporaries
+    sort_h.assign((o_keys - o_chunks) / sizeof(int), 0);
+    if (!chunks.empty()) std::memcpy(sort_h.data(), chunks.data(), sizeof(int4) * chunks.size());
+    std::memcpy(sort_h.data() + o_base / sizeof(int), base.data(), sizeof(int) * ncurves);
+    std::memcpy(sort_h.data() + o_nch / sizeof(int), nch.data(), sizeof(int) * ncurves);
+    CUNA_HIP(hipMemcpyAsync(tb, sort_h.data(), sizeof(int) * sort_h.size(), hipMemcpyHostToDevice,
                             s));
-    CUNA_HIP(hipMemcpyAsync(d_nch.ptr, nch.data(), sizeof(int) * ncurves, hipMemcpyHostToDevice,
-                            s));
+    int *const keys = reinterpret_cast<int *>(tb + o_keys);
+    PointRec<T> *const recA = reinterpret_cast<PointRec<T> *>(tb + o_recA);
+    int *const permA = reinterpret_cast<int *>(tb + o_permA);
     SortWork W;
-    W.chunks = d_chunks.as<int4>();
-    W.chunk_base = d_base.as<int>();
-    W.nch = d_nch.as<int>();
+    W.chunks = reinterpret_cast<const int4 *>(tb + o_chunks);
+    W.chunk_base = reinterpret_cast<const int *>(tb + o_base);
+    W.nch = reinterpret_cast<const int *>(tb + o_nch);
     W.coff = coff.as<int>();
     W.nchunks = (int)chunks.size();
     W.ncurves = ncurves;
     W.total_chunks = total_chunks;
-    W.hist = hist.as<int>();
-    W.bsum = bsum.as<int>();
+    W.hist = reinterpret_cast<int *>(tb + o_hist);
+    W.bsum = reinterpret_cast<int *>(tb + o_bsum);
     // phase reduction / compat decisions, then an LSD sort by anchor: the cell
     // within the tile, then the tile index in digits of at most 8 bits; the
     // passes alternate between rec/perm and recA/permA, ending in rec/perm
@@ -177,19 +221,19 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
     A.G = G;
     A.tlog = tlog;
     A.rec = rec_orig.as<PointRec<T>>();
-    A.key = keys.as<int>();
+    A.key = keys;
     launch_point_setup<T>(A, s);
     const int tbits = ilog2(ntiles);
     const int npass = 1 + (tbits + 7) / 8;
-    PointRec<T> *const rbuf[2] = {rec.as<PointRec<T>>(), recA.as<PointRec<T>>()};
-    int *const pbuf[2] = {perm.as<int>(), permA.as<int>()};
+    PointRec<T> *const rbuf[2] = {rec.as<PointRec<T>>(), recA};
+    int *const pbuf[2] = {perm.as<int>(), permA};
     auto dst = [&](int p) { return (npass - 1 - p) & 1; };
-    launch_sort_pass<T>(W, K1, keys.as<int>(), rec_orig.as<PointRec<T>>(), nullptr, rbuf[dst(0)],
+    launch_sort_pass<T>(W, K1, keys, rec_orig.as<PointRec<T>>(), nullptr, rbuf[dst(0)],
                         pbuf[dst(0)], s);
     for (int p = 1; p < npass; ++p) {
         const int shift = 8 * (p - 1), bits = std::min(8, tbits - shift);
-        launch_digit_keys<T>(rbuf[dst(p - 1)], n, tlog + shift, (1 << bits) - 1, keys.as<int>(), s);
-        launch_sort_pass<T>(W, 1 << bits, keys.as<int>(), rbuf[dst(p - 1)], pbuf[dst(p - 1)],
+        launch_digit_keys<T>(rbuf[dst(p - 1)], n, tlog + shift, (1 << bits) - 1, keys, s);
+        launch_sort_pass<T>(W, 1 << bits, keys, rbuf[dst(p - 1)], pbuf[dst(p - 1)],
                             rbuf[dst(p)], pbuf[dst(p)], s);
     }
     launch_buckets<T>(coff.as<int>(), ncurves, Kc, rec.as<PointRec<T>>(), tlog, olo,
@@ -208,27 +252,10 @@ void TiledPoints<T>::build(SetupArgs<T> A, const std::vector<int> &offsets, int 
         C.ntaps = ntaps;
         C.periodic = periodic ? 1 : 0;
         C.wrap1 = (periodic && ntiles == 1) ? 1 : 0;
-        const size_t ncov = (size_t)std::max(ncurves, 1) * std::max(ntiles, 1);
-        tile_cover.alloc(sizeof(int) * ncov);
         launch_tile_cover<T>(C, ncurves, tile_cover.as<int>(), s);
-        tile_desc.alloc(sizeof(TileDesc) * ncov);
         launch_tile_order(bucket.as<int>(), Kc, ntiles, ncurves, periodic, tile_cover.as<int>(),
                           tile_desc.as<TileDesc>(), s);
     }
-    // work-queue order of the curves for the fused batched kernels
-    std::vector<int> ord(ncurves);
-    for (int c = 0; c < ncurves; ++c) ord[c] = c;
-    std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
-        return offsets[u + 1] - offsets[u] > offsets[v + 1] - offsets[v];
-    });
-    order_h = std::move(ord);
-    order.alloc(sizeof(int) * ncurves);
-    CUNA_HIP(hipMemcpyAsync(order.ptr, order_h.data(), sizeof(int) * ncurves,
-                            hipMemcpyHostToDevice, s));
-    // the host tables stay alive with the plan: the copies above are async
-    chunks_h = std::move(chunks);
-    base_h = std::move(base);
-    nch_h = std::move(nch);
 }
 
 // --------------------------------------------------------------- Plan --

@@ -494,7 +494,10 @@ void launch_tile_order(const int *bucket, int Kc, int ntiles, int ncurves, bool 
     require(ntiles <= kOrderMaxTiles, "too many spread tiles for the work order",
             CUNA_ERROR_UNSUPPORTED);
     hipLaunchKernelGGL(k_tile_order, dim3(ncurves), dim3(kOrderThreads), 0, s, bucket, Kc, ntiles,
-                       periodic ? 1 : 0, cover, desc);
+                       desc);
+    CUNA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tile_desc, dim3(ceil_div(ntiles, 256) * ncurves), dim3(256), 0, s, bucket,
+                       Kc, ntiles, periodic ? 1 : 0, cover, desc);
     CUNA_LAUNCH_CHECK();
 }
 

@@ -16,6 +16,7 @@ namespace cuna {
 struct DevMem {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool owned = true;  // false: a view into another allocation
     DevMem() = default;
     DevMem(const DevMem &) = delete;
     DevMem &operator=(const DevMem &) = delete;
@@ -24,13 +25,20 @@ struct DevMem {
         if (nbytes) CUNA_HIP(hipMalloc(&ptr, nbytes));
         bytes = nbytes;
     }
+    void view(void *p, size_t nbytes) {
+        release();
+        ptr = p;
+        bytes = nbytes;
+        owned = false;
+    }
     void ensure(size_t nbytes) {
         if (nbytes > bytes) alloc(nbytes);
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && owned) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
+        owned = true;
     }
     ~DevMem() { release(); }
     template <typename U>
@@ -42,17 +50,16 @@ struct DevMem {
 // Points of one or more curves binned by grid tile (stable counting sort).
 template <typename T>
 struct TiledPoints {
-    // host copies of the sort tables, declared first so that they are
-    // destroyed after the device buffers (whose hipFree synchronises)
-    std::vector<int4> chunks_h;
-    std::vector<int> base_h;
-    std::vector<int> nch_h;
-    std::vector<int> order_h;
+    // host sources of the async uploads (offsets and curve order; the sort's
+    // chunk tables), declared first so that they are destroyed after the
+    // device buffers (whose hipFree synchronises)
+    std::vector<int> plan_h, sort_h;
     int ncurves = 0;
     long n = 0;
     std::vector<int> coff_h;
     int G = 0, tlog = 0, ntiles = 0, Kc = 0, olo = 0, ntaps = 0;
     bool periodic = true;
+    DevMem arena;  // one allocation; the buffers below are views into it
     DevMem coff, rec, perm, bucket, eps_orig;
     DevMem rec_orig;  // (anchor, e) by original index
     DevMem order;     // curves by decreasing length (work-queue order)

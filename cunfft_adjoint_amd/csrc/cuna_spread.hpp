@@ -756,7 +756,8 @@ __global__ __launch_bounds__(256) void k_tile_cover(SpreadArgs<T> A, int *cover)
 // Work order of the scatter kernels, one workgroup per curve: the curve's
 // tiles by decreasing own point count, as a stable counting sort over
 // kOrderBins bins of the count (exact below 128 points, in steps of 8 up to
-// 1143, one bin above), ties in tile order; then one TileDesc per position.
+// 1143, one bin above), ties in tile order; position i's TileDesc gets its
+// tile, and k_tile_desc fills in the rest.
 // The order only schedules the tiles: every tile's cells come out the same
 // in any order.  Each of the 16 waves ranks a contiguous segment of tiles.
 constexpr int kOrderThreads = 1024, kOrderWaves = kOrderThreads / 64, kOrderBins = 256;
@@ -769,9 +770,7 @@ __device__ __forceinline__ int order_bin(int npts) {
 }
 
 static __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int *bucket, int Kc,
-                                                                     int ntiles, int periodic,
-                                                                     const int *cover,
-                                                                     TileDesc *desc) {
+                                                                     int ntiles, TileDesc *desc) {
     __shared__ int cnt[kOrderBins * kOrderWaves];  // [bin][wave]: counts, then starts
     __shared__ int scan_sh[kOrderThreads];
     __shared__ unsigned char bin_sh[kOrderMaxTiles];
@@ -780,8 +779,21 @@ static __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int *
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int *bk = bucket + (long)c * (Kc + 1);
     for (int i = tid; i < kOrderBins * kOrderWaves; i += kOrderThreads) cnt[i] = 0;
-    for (int t = tid; t < ntiles; t += kOrderThreads)
-        bin_sh[t] = (unsigned char)order_bin(bk[3 * t + 3] - bk[3 * t]);
+    // every thread's loads first (clamped to in-bounds tiles), then their
+    // uses, so that the loads are in flight together
+    constexpr int TPT = kOrderMaxTiles / kOrderThreads;
+    int lo[TPT], hi[TPT];
+#pragma unroll
+    for (int r = 0; r < TPT; ++r) {
+        const int tc = min(r * kOrderThreads + tid, ntiles - 1);
+        lo[r] = bk[3 * tc];
+        hi[r] = bk[3 * tc + 3];
+    }
+#pragma unroll
+    for (int r = 0; r < TPT; ++r) {
+        const int t = r * kOrderThreads + tid;
+        if (t < ntiles) bin_sh[t] = (unsigned char)order_bin(hi[r] - lo[r]);
+    }
     __syncthreads();
     const int seg = (ntiles + kOrderWaves - 1) / kOrderWaves;
     const int s0 = min(ntiles, wave * seg), s1 = min(ntiles, s0 + seg);
@@ -827,29 +839,33 @@ static __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int *
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < ntiles; i += kOrderThreads) {
-        const int t = order[i];
-        TileDesc d;
-        d.tile = t;
-        d.b1 = bk[3 * t];
-        d.e1 = bk[3 * t + 3];
-        int prev = t - 1, next = t + 1;
-        bool hp = true, hn = true;
-        if (prev < 0) {
-            prev = ntiles - 1;
-            hp = periodic != 0;
-        }
-        if (next >= ntiles) {
-            next = 0;
-            hn = periodic != 0;
-        }
-        d.b0 = hp ? bk[3 * prev + 2] : 0;
-        d.e0 = hp ? bk[3 * prev + 3] : 0;
-        d.b2 = hn ? bk[3 * next] : 0;
-        d.e2 = hn ? bk[3 * next + 1] : 0;
-        d.cover = cover ? cover[(long)c * ntiles + t] : 0x7fffffff;
-        desc[(long)c * ntiles + i] = d;
-    }
+    for (int i = tid; i < ntiles; i += kOrderThreads) desc[(long)c * ntiles + i].tile = order[i];
+}
+
+// The rest of each TileDesc: tile t's own range and its neighbours' edge
+// classes (prev = t - 1 and next = t + 1, wrapping when periodic).  Many
+// workgroups share the random reads of the bucket table.
+static __global__ __launch_bounds__(256) void k_tile_desc(const int *bucket, int Kc, int ntiles,
+                                                          int periodic, const int *cover,
+                                                          TileDesc *desc) {
+    const int nblk = (ntiles + 255) / 256;  // workgroups per curve
+    const int c = blockIdx.x / nblk, i = (blockIdx.x % nblk) * 256 + threadIdx.x;
+    if (i >= ntiles) return;
+    const int *bk = bucket + (long)c * (Kc + 1);
+    TileDesc *dp = desc + (long)c * ntiles + i;
+    const int t = dp->tile;
+    const int prev = t > 0 ? t - 1 : ntiles - 1, next = t + 1 < ntiles ? t + 1 : 0;
+    const bool hp = t > 0 || periodic != 0, hn = t + 1 < ntiles || periodic != 0;
+    TileDesc d;
+    d.tile = t;
+    d.b1 = bk[3 * t];
+    d.e1 = bk[3 * t + 3];
+    d.b0 = hp ? bk[3 * prev + 2] : 0;
+    d.e0 = hp ? bk[3 * prev + 3] : 0;
+    d.b2 = hn ? bk[3 * next] : 0;
+    d.e2 = hn ? bk[3 * next + 1] : 0;
+    d.cover = cover ? cover[(long)c * ntiles + t] : 0x7fffffff;
+    *dp = d;
 }
 
 // Accumulators of the scatter: 32-bit cells (ds_add_u32) hold sums within
