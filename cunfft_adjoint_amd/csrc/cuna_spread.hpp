@@ -425,35 +425,6 @@ __device__ __forceinline__ void tap_weights(T v, T e, T bi, const T *e3, T *w) {
     }
 }
 
-// The same weights by two ratio recurrences, which need exp(-bi) instead of
-// the whole E3 table (16 fewer scalar registers): with w_o = v
-// exp(-(o-e)^2 bi), w_{o+1} = w_o R_o with R_0 = E2 exp(-bi),
-// R_{o+1} = R_o exp(-2 bi); downwards w_{o-1} = w_o L_o with
-// L_0 = exp(-bi)/E2, L_{o-1} = L_o exp(-2 bi).
-template <typename T, int NTAPS>
-__device__ __forceinline__ void tap_weights_rec(T v, T e, T bi, T c1, T *w) {
-    constexpr int Q0 = (NTAPS - 1) / 2;
-    const T D = c1 * c1;
-    const T E1 = dexp(-e * e * bi);
-    const T E2 = dexp(T(2) * e * bi);
-    const T E2i = dexp(T(-2) * e * bi);
-    const T w0 = v * E1;
-    T up = w0, r = E2 * c1;
-#pragma unroll
-    for (int q = Q0; q < NTAPS; ++q) {
-        w[q] = up;
-        up *= r;
-        r *= D;
-    }
-    T dn = w0, l = E2i * c1;
-#pragma unroll
-    for (int q = Q0 - 1; q >= 0; --q) {
-        dn *= l;
-        l *= D;
-        w[q] = dn;
-    }
-}
-
 // value and weight parameter of sorted point s (original index p)// value and weight parameter of sorted point s (original index p)
 template <typename T, int VM>
 __device__ __forceinline__ void point_value(const SpreadArgs<T> &A, int p, int c, int rep, T &v,
@@ -902,15 +873,16 @@ struct Fixed32 {
 };
 
 // The NTAPS fixed-point weights of one point, rounded to nearest:
-// n_q = round(v exp(-(q - Q0 - e)^2 bi)), v already carrying the scale.  In
-// FP32 with the table e3 (accurate filters), each tap is one multiply (the
-// step E2^{+-1} outward from the anchor), one fma that also adds the rounding
-// half (with the sign of v: all taps of a point share it) and one truncating
-// conversion.  Otherwise the weights by ratio recurrences (tap_weights_rec,
-// which needs only c1 = exp(-bi)) and FX::fixed per tap.
-template <typename T, int NTAPS, class FX, bool TABLE>
-__device__ __forceinline__ void tap_fixed(T v, T e, T bi, const T *e3, T c1,
-                                          typename FX::U *n) {
+// n_q = round(v exp(-(q - Q0 - e)^2 bi)), v already carrying the scale, each
+// handed to emit(q, n_q) as soon as it is formed (outward from the anchor),
+// so that no array of NTAPS values stays live.  In FP32 with the table e3
+// (accurate filters), each tap is one multiply (the step E2^{+-1}), one fma
+// that also adds the rounding half (with the sign of v: all taps of a point
+// share it) and one truncating conversion; FP64 rounds by FX::fixed_fma.
+// Otherwise (compat filters) the weights come by ratio recurrences, which
+// need only c1 = exp(-bi), and FX::fixed per tap.
+template <typename T, int NTAPS, class FX, bool TABLE, class Emit>
+__device__ __forceinline__ void tap_fixed(T v, T e, T bi, const T *e3, T c1, Emit &&emit) {
     constexpr int Q0 = (NTAPS - 1) / 2;
     if constexpr (TABLE && sizeof(T) == 4 && std::is_same<FX, Fixed32>::value) {
         const float E1 = dexp(-e * e * bi);
@@ -920,40 +892,60 @@ __device__ __forceinline__ void tap_fixed(T v, T e, T bi, const T *e3, T c1,
         float up = v * E1;
 #pragma unroll
         for (int q = Q0; q < NTAPS; ++q) {
-            n[q] = (unsigned)(int)fmaf(up, e3[q], h);
+            emit(q, (unsigned)(int)fmaf(up, e3[q], h));
             up *= E2;
         }
         float dn = v * E1 * E2i;
 #pragma unroll
         for (int q = Q0 - 1; q >= 0; --q) {
-            n[q] = (unsigned)(int)fmaf(dn, e3[q], h);
+            emit(q, (unsigned)(int)fmaf(dn, e3[q], h));
             dn *= E2i;
         }
     } else if constexpr (TABLE && sizeof(T) == 8 && std::is_same<FX, Fixed64<T>>::value) {
-        // the same walk in FP64, each tap rounded by FX::fixed_fma
         const double E1 = dexp(-e * e * bi);
         const double E2 = dexp(2.0 * e * bi);
         const double E2i = dexp(-2.0 * e * bi);
         double up = v * E1;
 #pragma unroll
         for (int q = Q0; q < NTAPS; ++q) {
-            n[q] = FX::fixed_fma(up, e3[q]);
+            emit(q, FX::fixed_fma(up, e3[q]));
             up *= E2;
         }
         double dn = v * E1 * E2i;
 #pragma unroll
         for (int q = Q0 - 1; q >= 0; --q) {
-            n[q] = FX::fixed_fma(dn, e3[q]);
+            emit(q, FX::fixed_fma(dn, e3[q]));
             dn *= E2i;
         }
-    } else {
+    } else if constexpr (TABLE) {
         T w[NTAPS];
-        if constexpr (TABLE)
-            tap_weights<T, NTAPS>(v, e, bi, e3, w);
-        else
-            tap_weights_rec<T, NTAPS>(v, e, bi, c1, w);
+        tap_weights<T, NTAPS>(v, e, bi, e3, w);
 #pragma unroll
-        for (int q = 0; q < NTAPS; ++q) n[q] = FX::fixed(w[q]);
+        for (int q = 0; q < NTAPS; ++q) emit(q, FX::fixed(w[q]));
+    } else {
+        // two ratio recurrences, which need exp(-bi) instead of the table:
+        // with w_o = v exp(-(o-e)^2 bi), w_{o+1} = w_o R_o, R_0 = E2 exp(-bi),
+        // R_{o+1} = R_o exp(-2 bi); downwards w_{o-1} = w_o L_o with
+        // L_0 = exp(-bi)/E2, L_{o-1} = L_o exp(-2 bi)
+        const T D = c1 * c1;
+        const T E1 = dexp(-e * e * bi);
+        const T E2 = dexp(T(2) * e * bi);
+        const T E2i = dexp(T(-2) * e * bi);
+        const T w0 = v * E1;
+        T up = w0, r = E2 * c1;
+#pragma unroll
+        for (int q = Q0; q < NTAPS; ++q) {
+            emit(q, FX::fixed(up));
+            up *= r;
+            r *= D;
+        }
+        T dn = w0, l = E2i * c1;
+#pragma unroll
+        for (int q = Q0 - 1; q >= 0; --q) {
+            dn *= l;
+            l *= D;
+            emit(q, FX::fixed(dn));
+        }
     }
 }
 
@@ -1097,21 +1089,20 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
                 if (k * NT + tid < cnt) {
-                    Acc nq[NTAPS];
                     // accurate filters (even NTAPS) share one table e3; compat
                     // batches carry a filter per curve (cpar), so recurrences
-                    tap_fixed<T, NTAPS, FX, NTAPS % 2 == 0>(vv[k] * mul, ev[k], bi, A.e3, c1, nq);
                     const int c0 = am[k] - Q0;  // first tap's cell
+                    Acc *dst = acc + HALO + c0;
                     if (R.tiny) {
-#pragma unroll
-                        for (int q = 0; q < NTAPS; ++q) {
-                            const int cell = c0 + q;
-                            if (cell >= -HALO && cell < TT + HALO) atomicAdd(&acc[HALO + cell], nq[q]);
-                        }
+                        tap_fixed<T, NTAPS, FX, NTAPS % 2 == 0>(
+                            vv[k] * mul, ev[k], bi, A.e3, c1, [&](int q, Acc n) {
+                                const int cell = c0 + q;
+                                if (cell >= -HALO && cell < TT + HALO) atomicAdd(dst + q, n);
+                            });
                     } else {
-                        Acc *dst = acc + HALO + c0;
-#pragma unroll
-                        for (int q = 0; q < NTAPS; ++q) atomicAdd(dst + q, nq[q]);
+                        tap_fixed<T, NTAPS, FX, NTAPS % 2 == 0>(
+                            vv[k] * mul, ev[k], bi, A.e3, c1,
+                            [&](int q, Acc n) { atomicAdd(dst + q, n); });
                     }
                 }
             }
