@@ -1363,12 +1363,22 @@ __global__ __launch_bounds__(NT) void k_boot_fused(BootArgs<T> A) {
 #pragma unroll
                     for (int k = 0; k < PB; ++k) {
                         if (p0 + k * NT + tid < nc) {
-                            T w[NTAPS];
-                            tap_weights<T, NTAPS>(vv[k] * mul, pr[k].e, A.bi, A.e3, w);
                             const int base = pr[k].a - Q0;
+                            if constexpr (sizeof(T) == 4) {
+                                // each tap's rounded weight to its atomic as
+                                // it is formed (weights <= 2^headroom_bits)
+                                tap_fixed<T, NTAPS, Fixed32, true>(
+                                    vv[k] * mul, pr[k].e, A.bi, A.e3, T(0), [&](int q, unsigned n) {
+                                        atomicAdd(&gu[(base + q) & (G - 1)], (U)n);
+                                    });
+                            } else {
+                                T w[NTAPS];
+                                tap_weights<T, NTAPS>(vv[k] * mul, pr[k].e, A.bi, A.e3, w);
 #pragma unroll
-                            for (int q = 0; q < NTAPS; ++q)
-                                atomicAdd(&gu[(base + q) & (G - 1)], (U)FixedOf<T>::round(w[q]));
+                                for (int q = 0; q < NTAPS; ++q)
+                                    atomicAdd(&gu[(base + q) & (G - 1)],
+                                              (U)FixedOf<T>::round(w[q]));
+                            }
                             atomicAdd(&hist_sh[pr[k].a >> 5], 1);
                         }
                     }
