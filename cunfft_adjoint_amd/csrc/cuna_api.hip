@@ -364,8 +364,7 @@ void Plan<T>::adjoint_many(const T *y, int nrhs, cplx<T> *out, hipStream_t s) {
     require(y != nullptr && out != nullptr, "NULL pointer");
     require(nrhs >= 1, "nrhs must be positive");
     const int nc = ncurves();
-    require((long)nc * nrhs <= 65535, "ncurves * nrhs must be at most 65535",
-            CUNA_ERROR_UNSUPPORTED);
+    require((long)nc * nrhs <= INT_MAX, "ncurves * nrhs too large", CUNA_ERROR_UNSUPPORTED);
     grid.ensure(sizeof(T) * (size_t)G * nc * nrhs);
     SpreadArgs<T> A = spread_args(pts);
     A.y = y;

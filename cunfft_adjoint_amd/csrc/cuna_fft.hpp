@@ -581,7 +581,7 @@ __global__ __launch_bounds__(NT) void k_fft_small(Loader ld, Sink sk, cplx<T> *p
 // w_N^(S n1 r2), store at n1 + L1 r2 of dst (may alias the source).
 template <typename T, int S, int LG2, int W, int NT, class Loader, int MR = 4>
 __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long dst_stride, int L1,
-                                                const cplx<T> *tw, Tw2<T> twN) {
+                                                const cplx<T> *tw, Tw2<T> twN, long b0) {
     constexpr int L2 = 1 << LG2;
     constexpr bool TAB = Loader::kStage && lds_table(LG2);
     // epilogue twiddles held through the FFT only where registers are ample:
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
     const int tid = threadIdx.x;
     const int nblk = L1 / W;
     const int blk = xcd_remap(blockIdx.x, nblk);
-    const long b = blockIdx.y;
+    const long b = b0 + blockIdx.y;  // launches span at most kMaxGridY batches
     const int n10 = blk * W;
     Loader lb = ld;
     lb.advance(b);
@@ -644,7 +644,7 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
 template <typename T, int LG1, int H, int NT, class Sink, int MR = 4>
 __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long src_stride,
                                                      int L2, Sink sk, const cplx<T> *tw,
-                                                     Tw2<T> twG) {
+                                                     Tw2<T> twG, long b0) {
     constexpr int L1 = 1 << LG1;
     constexpr int W = 2 * H;
     constexpr bool TAB = lds_table(LG1);
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
     const int tid = threadIdx.x;
     const int nblk = L2 / (2 * H);
     const int i = xcd_remap(blockIdx.x, nblk);
-    const long b = blockIdx.y;
+    const long b = b0 + blockIdx.y;  // launches span at most kMaxGridY batches
     const long N = (long)L1 * L2;
     const cplx<T> *z = src + b * src_stride;
     TableStage<T, (TAB ? L1 : 1), NT> tab;
@@ -719,7 +719,7 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
 template <typename T, int S, int LG1, int H, int NT>
 __global__ __launch_bounds__(NT) void k_fft_row_plain(const cplx<T> *src, long src_stride,
                                                       int L2, cplx<T> *dst, long dst_stride,
-                                                      const cplx<T> *tw) {
+                                                      const cplx<T> *tw, long b0) {
     constexpr int L1 = 1 << LG1;
     constexpr bool TAB = lds_table(LG1);
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(NT) void k_fft_row_plain(const cplx<T> *src, long s
     const int tid = threadIdx.x;
     const int nblk = L2 / H;
     const int i = xcd_remap(blockIdx.x, nblk);
-    const long b = blockIdx.y;
+    const long b = b0 + blockIdx.y;  // launches span at most kMaxGridY batches
     const cplx<T> *z = src + b * src_stride;
     TableStage<T, (TAB ? L1 : 1), NT> tab;
     if constexpr (TAB) tab.load(tw, tid);

@@ -82,9 +82,12 @@ static void run_col(const FftShape &sh, const Loader &ld, cplx<T> *dst, int L1, 
     auto k = k_fft_col<T, S, LG2, W, NT, Loader, kColMr>;
     set_lds(k, lds);
     require(L1 % W == 0, "fft column split");
-    hipLaunchKernelGGL(k, dim3(L1 / W, sh.batch), dim3(NT), lds, s, ld, dst, N, L1, tc.fine(LG2),
-                       tc.two_level(sh.lgN));
-    CUNA_LAUNCH_CHECK();
+    for (long b0 = 0; b0 < sh.batch; b0 += kMaxGridY) {
+        const int nb = (int)std::min<long>(kMaxGridY, sh.batch - b0);
+        hipLaunchKernelGGL(k, dim3(L1 / W, nb), dim3(NT), lds, s, ld, dst, N, L1, tc.fine(LG2),
+                           tc.two_level(sh.lgN), b0);
+        CUNA_LAUNCH_CHECK();
+    }
 }
 
 template <typename T, int LG1, int H, class Sink>
@@ -97,9 +100,12 @@ static void run_row_real(const FftShape &sh, const cplx<T> *src, int L2, const S
     auto k = k_fft_row_real<T, LG1, H, NT, Sink, kRowMr>;
     set_lds(k, lds);
     require(L2 % (2 * H) == 0, "fft row split");
-    hipLaunchKernelGGL(k, dim3(L2 / (2 * H), sh.batch), dim3(NT), lds, s, src, N, L2, sk,
-                       tc.fine(LG1), tc.two_level(sh.lgN + 1));
-    CUNA_LAUNCH_CHECK();
+    for (long b0 = 0; b0 < sh.batch; b0 += kMaxGridY) {
+        const int nb = (int)std::min<long>(kMaxGridY, sh.batch - b0);
+        hipLaunchKernelGGL(k, dim3(L2 / (2 * H), nb), dim3(NT), lds, s, src, N, L2, sk,
+                           tc.fine(LG1), tc.two_level(sh.lgN + 1), b0);
+        CUNA_LAUNCH_CHECK();
+    }
 }
 
 template <typename T, int S, int LG1, int H>
@@ -112,9 +118,12 @@ static void run_row_plain(const FftShape &sh, const cplx<T> *src, int L2, cplx<T
     auto k = k_fft_row_plain<T, S, LG1, H, NT>;
     set_lds(k, lds);
     require(L2 % H == 0, "fft row split");
-    hipLaunchKernelGGL(k, dim3(L2 / H, sh.batch), dim3(NT), lds, s, src, N, L2, dst, N,
-                       tc.fine(LG1));
-    CUNA_LAUNCH_CHECK();
+    for (long b0 = 0; b0 < sh.batch; b0 += kMaxGridY) {
+        const int nb = (int)std::min<long>(kMaxGridY, sh.batch - b0);
+        hipLaunchKernelGGL(k, dim3(L2 / H, nb), dim3(NT), lds, s, src, N, L2, dst, N, tc.fine(LG1),
+                           b0);
+        CUNA_LAUNCH_CHECK();
+    }
 }
 
 // Sequences per workgroup of a pass: the largest count whose LDS tile fits in
@@ -412,6 +421,20 @@ static void launch_timed(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s, 
     CUNA_LAUNCH_CHECK();
 }
 
+// grid (ntiles, slabs) in launches of at most kMaxGridY slabs; ev0 records
+// the first launch's start, ev1 the last one's stop
+template <typename K, typename T>
+static void launch_slabs(K k, const SpreadArgs<T> &A, int nslabs, dim3 block, size_t lds,
+                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    for (int s0 = 0; s0 < nslabs; s0 += kMaxGridY) {
+        const int ns = std::min(kMaxGridY, nslabs - s0);
+        SpreadArgs<T> B = A;
+        B.slab0 = s0;
+        launch_timed(k, dim3(A.ntiles, ns), block, lds, s, s0 == 0 ? ev0 : nullptr,
+                     s0 + ns == nslabs ? ev1 : nullptr, B);
+    }
+}
+
 // The scatter form (default): one wave per tile, fixed-point cells per wave
 // (32-bit in FP32, 64-bit in FP64), nothing measured at plan time needed to be
 // safe.  CUNA_PLAN_GATHER selects the gather form.
@@ -424,14 +447,14 @@ static void spread_one(const SpreadArgs<T> &A, int nslabs, hipStream_t s, hipEve
         require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
         auto k = k_spread_wv<T, NTAPS, VM, WPT>;
         set_lds(k, lds);
-        launch_timed(k, dim3(A.ntiles, nslabs), dim3(64 * WPT), lds, s, ev0, ev1, A);
+        launch_slabs(k, A, nslabs, dim3(64 * WPT), lds, s, ev0, ev1);
         return;
     }
     const size_t lds = spread_lds_bytes<T>(A.tlog, NTAPS, A.cap);
     require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
     auto k = k_spread<T, NTAPS, VM>;
     set_lds(k, lds);
-    launch_timed(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, ev0, ev1, A);
+    launch_slabs(k, A, nslabs, dim3(256), lds, s, ev0, ev1);
 }
 
 // accurate filters have 2m taps, m in [2, kMaxTaps/2]; compat filters have
@@ -484,8 +507,13 @@ void launch_tile_cover(const SpreadArgs<T> &A, int ncurves, int *cover, hipStrea
     const size_t lds = sizeof(int) * (((size_t)1 << A.tlog) + 2 * (size_t)A.ntaps);
     require(lds <= kMaxLds, "spread tile too large", CUNA_ERROR_UNSUPPORTED);
     set_lds(k_tile_cover<T>, lds);
-    hipLaunchKernelGGL(k_tile_cover<T>, dim3(A.ntiles, ncurves), dim3(256), lds, s, A, cover);
-    CUNA_LAUNCH_CHECK();
+    for (int c0 = 0; c0 < ncurves; c0 += kMaxGridY) {  // blockIdx.y = curve - slab0
+        SpreadArgs<T> B = A;
+        B.slab0 = c0;
+        hipLaunchKernelGGL(k_tile_cover<T>, dim3(A.ntiles, std::min(kMaxGridY, ncurves - c0)),
+                           dim3(256), lds, s, B, cover);
+        CUNA_LAUNCH_CHECK();
+    }
 }
 
 void launch_tile_order(const int *bucket, int Kc, int ntiles, int ncurves, bool periodic,
@@ -509,8 +537,13 @@ void launch_gather(const GatherArgs<T> &A, int ntaps, int nslabs, hipStream_t s)
         size_t lds = sizeof(T) * (((size_t)1 << A.tlog) + NTP);
         auto k = k_gather<T, NTP>;
         set_lds(k, lds);
-        hipLaunchKernelGGL(k, dim3(A.ntiles, nslabs), dim3(256), lds, s, A);
-        CUNA_LAUNCH_CHECK();
+        for (int s0 = 0; s0 < nslabs; s0 += kMaxGridY) {
+            GatherArgs<T> B = A;
+            B.slab0 = s0;
+            hipLaunchKernelGGL(k, dim3(A.ntiles, std::min(kMaxGridY, nslabs - s0)), dim3(256),
+                               lds, s, B);
+            CUNA_LAUNCH_CHECK();
+        }
     });
 }
 

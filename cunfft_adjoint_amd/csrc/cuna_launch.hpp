@@ -101,5 +101,7 @@ void launch_boot_fused(BootArgs<T> A, int lgN, int ntaps, int mode, hipStream_t 
 
 // bytes of dynamic LDS a launch may use on gfx950
 constexpr size_t kMaxLds = 160 * 1024;
+// the largest grid y dimension; launches over more slabs or batches split
+constexpr int kMaxGridY = 65535;
 
 }  // namespace cuna

@@ -375,6 +375,7 @@ struct SpreadArgs {
     T *grid;
     long grid_stride;      // per slab
     int G, tlog, ntiles, Kc, olo, ntaps, nrep;
+    int slab0;  // slab (plan-time kernels: curve) of blockIdx.y = 0
     int periodic;
     int wrap1;  // periodic and a single tile covers the grid
     int cap;    // points staged in LDS per round (gather form)
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> A) {
     // consecutive tiles go to different XCDs (hardware round-robin): dense
     // regions of clustered points are spread over all of them
     const int tile = blockIdx.x;
-    const int slab = blockIdx.y;
+    const int slab = A.slab0 + blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
     TileRanges R;
     R.init(A, tile, c, NTAPS);
@@ -693,7 +694,7 @@ __global__ __launch_bounds__(256) void k_tile_cover(SpreadArgs<T> A, int *cover)
     constexpr int NT = 256;
     int *hist = reinterpret_cast<int *>(cuna_smem);
     __shared__ int red[NT / 64];
-    const int tile = blockIdx.x, c = blockIdx.y;
+    const int tile = blockIdx.x, c = A.slab0 + blockIdx.y;
     const int TT = 1 << A.tlog, nt = A.ntaps, Q0 = (nt - 1) / 2, NH = TT + 2 * nt;
     TileRanges R;
     R.init(A, tile, c, nt);
@@ -996,7 +997,7 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
     Acc *acc = acc_all + wave * NC;
     T *facc = facc_all + wave * TT;
 
-    const int slab = blockIdx.y;
+    const int slab = A.slab0 + blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
     TileRanges R;
     int cover = CAP;
@@ -1620,6 +1621,7 @@ struct GatherArgs {
     long out_stride;    // per rep
     int dy_elem;        // element stride of dy (2: one part of an interleaved complex array)
     int G, tlog, ntiles, Kc, olo, nrep;
+    int slab0;  // slab of blockIdx.y = 0
     T bi, wscale, dxscale;  // dxscale = 2 G / beta
     T e3[kMaxTaps];
 };
@@ -1631,7 +1633,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs<T> A) {
     const int TT = 1 << A.tlog;
     T *hs = reinterpret_cast<T *>(cuna_smem);  // cells tstart-Q0 .. tstart+TT-Q0+NTAPS-1
     const int tile = xcd_remap(blockIdx.x, A.ntiles);
-    const int slab = blockIdx.y;
+    const int slab = A.slab0 + blockIdx.y;
     const int c = slab / A.nrep, rep = slab % A.nrep;
     const int *bk = A.bucket + (long)c * (A.Kc + 1);
     const int tstart = tile << A.tlog;

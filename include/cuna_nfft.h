@@ -91,7 +91,7 @@ int cuna_nfft_adjoint(cuna_nfft_plan plan, const dTyp *y, Complex *out, hipStrea
 /* accurate mode: nrhs value vectors of one plan in one spread launch and one
  * batched FFT.  y: nrhs x n (row r holds the values of right-hand side r, in
  * the plan's point order); out: ncurves x nrhs x M complex, row c*nrhs + r
- * equal to cuna_nfft_adjoint of row r for curve c.  ncurves * nrhs <= 65535.
+ * equal to cuna_nfft_adjoint of row r for curve c.
  * No counterpart in the reference (its batching is over bootstrap replicates,
  * src/cuna_kernel.cu:236-289). */
 int cuna_nfft_adjoint_many(cuna_nfft_plan plan, const dTyp *y, int nrhs, Complex *out,
