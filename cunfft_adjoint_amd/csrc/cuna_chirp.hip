@@ -2,7 +2,7 @@
 // powers of two, on top of the power-of-two Stockham FFT.
 //
 // The reference's cuFFT stage takes any ng (cufftPlan1d / cufftPlanMany,
-// src/cuna_kernel.cu:82,169,208,261; its test driver uses ng = R*N,
+// src/cuna_kernel.cu:82,169,208,261; its test program uses ng = R*N,
 // test.cpp:114).  Here a real grid of ng cells is transformed through
 //   A[r] = c_r * sum_g (z_g c_g) conj(c_{r-g}),  c_k = exp(+i pi k^2/ng),
 // with the convolution done circularly at L = 2^lgL >= 2 ng - 1.  The chirp

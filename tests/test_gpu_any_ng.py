@@ -2,7 +2,7 @@
 the C-ABI, against the oracle's restatement of the reference formula.
 
 The reference hands any ng to cuFFT (src/cuna_kernel.cu:82,169,208,261) and
-its own driver builds ng = R*N (test.cpp:114), e.g. 2 * 1000 = 2000.  Here
+its own test program builds ng = R*N (test.cpp:114), e.g. 2 * 1000 = 2000.  Here
 such lengths take the Bluestein transform (csrc/cuna_chirp.hip) over the
 power-of-two Stockham FFT.  Tolerances as for the power-of-two grids: rel L2
 1e-5 (libcunaf) / 1e-12 (libcunad).
@@ -48,7 +48,7 @@ def cdt(prec):
 @pytest.mark.parametrize("ng", [2000, 3000, 6144, 10000])
 def test_compat_entries_any_ng(prec, ng):
     """cunfft_adjoint (host arrays), cunfft_adjoint_raw_async and
-    cunfft_adjoint_raw_async_bootstrap at ng = 2 n (the reference driver's
+    cunfft_adjoint_raw_async_bootstrap at ng = 2 n (the reference test program's
     R = 2), against the oracle."""
     n = ng // 2
     x, y = uniform_points(n, ng, NP[prec])
