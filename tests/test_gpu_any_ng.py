@@ -45,11 +45,12 @@ def cdt(prec):
 
 
 @pytest.mark.parametrize("prec", ["f", "d"])
-@pytest.mark.parametrize("ng", [2000, 3000, 6144, 10000])
+@pytest.mark.parametrize("ng", [2000, 3000, 6144, 10000, 100000])
 def test_compat_entries_any_ng(prec, ng):
     """cunfft_adjoint (host arrays), cunfft_adjoint_raw_async and
     cunfft_adjoint_raw_async_bootstrap at ng = 2 n (the reference test program's
-    R = 2), against the oracle."""
+    R = 2), against the oracle.  ng = 100000: 391 spread tiles, a partial last
+    tile, and two sort passes over the tile index."""
     n = ng // 2
     x, y = uniform_points(n, ng, NP[prec])
     dbl = prec == "d"
