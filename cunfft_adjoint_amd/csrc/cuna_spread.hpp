@@ -1028,14 +1028,21 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
     const bool contig = R.contiguous();  // uniform
     for (int base = 0; base < total; base += CAP) {
         const int cnt = min(CAP, total - base);
-        // slot i of the round takes point i (loads coalesce; branch-free, so
-        // all slots issue together); a slot past the end reads point 0
+        // thread t takes the round's points t*spl .. t*spl + spl-1, spl the
+        // fewest slots that hold cnt points: a wave's lanes then sit on
+        // points spread over the whole tile, so one atomic instruction seldom
+        // puts two lanes on one cell (the points are sorted by anchor), and
+        // slots past spl are empty for every lane.  Branch-free loads (all
+        // slots issue together); a slot past the end reads point 0.
+        const int spl = (cnt + NT - 1) / NT;  // uniform
+        auto slot = [&](int k) { return k < spl ? tid * spl + k : CAP; };
         int am[PPT], pk[PPT];
         T ev[PPT], vv[PPT];
-        int wcnt = 0;  // this wave's points in the round
+        // this wave's points in the round
+        const int wcnt = max(0, min(64 * spl, cnt - wave * 64 * spl));
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
-            const int i = k * NT + tid;
+            const int i = slot(k);
             const int j = base + (i < cnt ? i : 0);
             int s, ro;
             if (contig) {
@@ -1048,14 +1055,13 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
             am[k] = pr.a - ro;
             ev[k] = pr.e;
             pk[k] = A.perm[s];
-            wcnt += max(0, min(64, cnt - (k * NT + wave * 64)));
         }
         T vmax = T(0);
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             T v, e = ev[k];
             point_value<T, VM>(A, pk[k], c, rep, v, e);
-            const bool ok = k * NT + tid < cnt;
+            const bool ok = slot(k) < cnt;
             vv[k] = ok ? v : T(0);
             ev[k] = e;
             T av = fabs(vv[k]);
@@ -1089,7 +1095,7 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
         if (finite) {
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
-                if (k * NT + tid < cnt) {
+                if (slot(k) < cnt) {
                     // accurate filters (even NTAPS) share one table e3; compat
                     // batches carry a filter per curve (cpar), so recurrences
                     const int c0 = am[k] - Q0;  // first tap's cell
