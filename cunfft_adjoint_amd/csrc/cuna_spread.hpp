@@ -1036,25 +1036,30 @@ __global__ __launch_bounds__(64 * WPT) void k_spread_wv(SpreadArgs<T> A) {
         // slots issue together); a slot past the end reads point 0.
         const int spl = (cnt + NT - 1) / NT;  // uniform
         auto slot = [&](int k) { return k < spl ? tid * spl + k : CAP; };
-        int am[PPT], pk[PPT];
+        int am[PPT], pk[PPT], sk[PPT], rk[PPT];
         T ev[PPT], vv[PPT];
         // this wave's points in the round
         const int wcnt = max(0, min(64 * spl, cnt - wave * 64 * spl));
+        // the original indices first, then the records: the values gathered
+        // through perm below wait only for perm (loads return in order), and
+        // their loads overlap the records'
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const int i = slot(k);
             const int j = base + (i < cnt ? i : 0);
-            int s, ro;
             if (contig) {
-                s = R.b0 + j;
-                ro = R.tstart;
+                sk[k] = R.b0 + j;
+                rk[k] = R.tstart;
             } else {
-                R.get(j, s, ro);
+                R.get(j, sk[k], rk[k]);
             }
-            const PointRec<T> pr = A.rec[s];
-            am[k] = pr.a - ro;
+            pk[k] = A.perm[sk[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const PointRec<T> pr = A.rec[sk[k]];
+            am[k] = pr.a - rk[k];
             ev[k] = pr.e;
-            pk[k] = A.perm[s];
         }
         T vmax = T(0);
 #pragma unroll
