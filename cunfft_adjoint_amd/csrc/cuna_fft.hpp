@@ -584,9 +584,10 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
                                                 const cplx<T> *tw, Tw2<T> twN, long b0) {
     constexpr int L2 = 1 << LG2;
     constexpr bool TAB = Loader::kStage && lds_table(LG2);
-    // epilogue twiddles held through the FFT only where registers are ample:
-    // up to 256 threads with radix 16, 128 with radix 8
-    constexpr bool PRE = Loader::kStage && (NT << (4 - MR)) <= 256 && sizeof(T) == 4;
+    // FP32: the epilogue twiddles are loaded as the first pass's hook, after
+    // its data loads (loads return in order, so the pass waits only for its
+    // data), and held through the FFT
+    constexpr bool PRE = Loader::kStage && sizeof(T) == 4;
     constexpr int OPT = PRE ? (W * L2 + NT - 1) / NT : 1;
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(cuna_smem);
     cplx<T> *twl = lds + W * L2;
@@ -600,20 +601,25 @@ __global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T> *dst, long ds
     TableStage<T, (TAB ? L2 : 1), NT> tab;
     if constexpr (TAB) tab.load(tw, tid);
     Tw2Parts<T> ot[OPT];
-    if constexpr (PRE) {
+    auto load_ot = [&]() __attribute__((always_inline)) {
+        if constexpr (PRE) {
 #pragma unroll
-        for (int i = 0; i < OPT; ++i) {
-            const int idx = tid + i * NT;
-            if ((W * L2) % NT == 0 || idx < W * L2)
-                ot[i].load(twN, (long)(n10 + idx % W) * (idx / W));
+            for (int i = 0; i < OPT; ++i) {
+                const int idx = tid + i * NT;
+                if ((W * L2) % NT == 0 || idx < W * L2)
+                    ot[i].load(twN, (long)(n10 + idx % W) * (idx / W));
+            }
         }
-    }
+    };
     auto fetch = [&](int e, int w) __attribute__((always_inline)) { return lb.at(n10 + w + (long)L1 * e); };
     if constexpr (TAB) {
-        auto hook = [&]() __attribute__((always_inline)) { tab.store(twl, tid); };
+        auto hook = [&]() __attribute__((always_inline)) {
+            tab.store(twl, tid);
+            load_ot();
+        };
         lds_fft<T, S, LG2, W, NT, MR>(lds, twl, fetch, hook);
     } else {
-        lds_fft<T, S, LG2, W, NT, MR>(lds, tw, fetch);
+        lds_fft<T, S, LG2, W, NT, MR>(lds, tw, fetch, load_ot);
     }
     cplx<T> *o = dst + b * dst_stride;
     if constexpr (PRE) {
@@ -660,14 +666,20 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
     TableStage<T, (TAB ? L1 : 1), NT> tab;
     if constexpr (TAB) tab.load(tw, tid);
     // epilogue twiddles for r = rho + L2 r1 (row 0 included; its mirror row
-    // L2/2 takes its own below)
+    // L2/2 takes its own below): in FP32 loaded as the first pass's hook,
+    // after its data loads, so that the pass waits only for its data (loads
+    // return in order); in FP64, where that measured slower, before them
     Tw2Parts<T> ot[OPT];
+    auto load_ot = [&]() __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < OPT; ++k) {
-        const int idx = tid + k * NT;
-        if ((H * L1) % NT == 0 || idx < H * L1)
-            ot[k].load(twG, (long)(i * H + idx % H) + (long)L2 * (idx / H));
-    }
+        for (int k = 0; k < OPT; ++k) {
+            const int idx = tid + k * NT;
+            if ((H * L1) % NT == 0 || idx < H * L1)
+                ot[k].load(twG, (long)(i * H + idx % H) + (long)L2 * (idx / H));
+        }
+    };
+    constexpr bool LATE = sizeof(T) == 4;
+    if constexpr (!LATE) load_ot();
     auto row_of = [&](int s) -> int {
         int h = s % H;
         int rho = i * H + h;
@@ -675,11 +687,17 @@ __global__ __launch_bounds__(NT) void k_fft_row_real(const cplx<T> *src, long sr
         return rho == 0 ? L2 / 2 : L2 - rho;
     };
     auto fetch = [&](int e, int s) __attribute__((always_inline)) { return z[(long)row_of(s) * L1 + e]; };
+    auto late = [&]() __attribute__((always_inline)) {
+        if constexpr (LATE) load_ot();
+    };
     if constexpr (TAB) {
-        auto hook = [&]() __attribute__((always_inline)) { tab.store(twl, tid); };
+        auto hook = [&]() __attribute__((always_inline)) {
+            tab.store(twl, tid);
+            late();
+        };
         lds_fft<T, 1, LG1, W, NT, MR>(lds, twl, fetch, hook);
     } else {
-        lds_fft<T, 1, LG1, W, NT, MR>(lds, tw, fetch);
+        lds_fft<T, 1, LG1, W, NT, MR>(lds, tw, fetch, late);
     }
     const auto skb = sk.for_row(b);
     cplx<T> *o = sk.row_out(b);
